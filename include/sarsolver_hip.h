@@ -1,0 +1,144 @@
+/*
+ * sarsolver_hip.h -- C ABI of the MI355X (gfx950) SAR forward-projection /
+ * backprojection library, libsarsolver_hip.so.
+ *
+ * Two families of entry points:
+ *
+ *  1. Reference-compatible entry points.  `forward_evaluate` and
+ *     `adjoint_evaluate` take the same 128-byte `SarCalculationInfo` record
+ *     (host memory, FP64, interleaved complex) that the reference binds with
+ *     ctypes, so the reference's loader can be pointed at this library
+ *     unchanged.  They always use the reference's nearest-neighbour bin rules
+ *     with FP64 ranges (interp = SARHIP_INTERP_NEAREST_REF,
+ *     precision = SARHIP_PRECISION_FP64).
+ *
+ *  2. Plan-based device API (`sarhip_*`).  Geometry is uploaded once into a
+ *     plan; forward / adjoint then run on device buffers (complex64, i.e.
+ *     interleaved float pairs) on a caller-supplied HIP stream.  This is what
+ *     the Python operator layer and the solvers call.
+ *
+ * Every entry point returns 0 on success and a negative SARHIP_E* code on
+ * failure; `sarhip_last_error()` gives the message for the calling thread.
+ * No C++ exception crosses this boundary.
+ */
+#ifndef SARSOLVER_HIP_H
+#define SARSOLVER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order, types and size (3*8 + 8*8 + 5*8 = 128 bytes) as the
+ * reference's record: src/sarsolver/cxx_binding/sarsolver_cxx.hpp:27-34,
+ * mirrored in ctypes at src/sarsolver/cxx_binding/sarsolver_cxx.py:53-62.
+ * Arrays are host memory: positions [P,3] / [M,3] FP64; complex arrays
+ * interleaved (re, im) FP64; phase_history [P,N] in FFT frequency order;
+ * waveform_fft [N] in FFT order. */
+typedef struct SarCalculationInfo {
+    size_t num_fast_times, num_slow_times, num_scatterers;
+    double *transmit_posns, *receive_posns, *stab_ref_posns, *scat_posns;
+    double *phase_history, *scattering_amplitudes, *waveform_fft, *slow_time_weighting;
+    double centre_frequency, sample_frequency, c_eff, upsample_ratio, sign_multiplier;
+} SarCalculationInfo;
+
+/* Error codes. */
+#define SARHIP_OK 0
+#define SARHIP_EINVAL (-1)  /* bad argument / inconsistent sizes            */
+#define SARHIP_EHIP (-2)    /* HIP runtime error                            */
+#define SARHIP_EFFT (-3)    /* rocFFT error                                 */
+#define SARHIP_ENOMEM (-4)  /* device or host allocation failed             */
+#define SARHIP_ENODEV (-5)  /* no usable gfx950 device                       */
+
+/* Fast-time interpolation rule. */
+#define SARHIP_INTERP_NEAREST_REF 0 /* reference rules: forward bin round(u),
+                                       adjoint bin round(0.5+u), adjoint does
+                                       not conjugate the slow-time weights   */
+#define SARHIP_INTERP_NEAREST 1     /* round(u) both ways; exact adjoint pair */
+#define SARHIP_INTERP_LINEAR 2      /* linear interpolation; exact adjoint pair */
+
+/* Range / phase arithmetic. */
+#define SARHIP_PRECISION_FP64 0   /* platform-to-pixel ranges in FP64 exactly as
+                                     the reference evaluates them             */
+#define SARHIP_PRECISION_SERIES 1 /* FP64 tile-centred series for the range;
+                                     exact FP64 re-evaluation at bin ties     */
+#define SARHIP_PRECISION_HILO 2   /* FP64 tile-centre range + FP32 differential
+                                     range from tile-relative coordinates     */
+#define SARHIP_PRECISION_FP32 3   /* all-FP32 absolute-coordinate ranges (study
+                                     only: fails at 10 km standoff)           */
+
+/* Phase-history frequency order of device buffers. */
+#define SARHIP_ORDER_FFT 0     /* FFT order (index 0 = zero frequency)       */
+#define SARHIP_ORDER_CENTRED 1 /* centred order (fftshift of FFT order), the
+                                  reference's public convention               */
+
+/* Replaces the reference's `forward_evaluate(SarCalculationInfo&)`
+ * (src/sarsolver/cxx_binding/sarsolver_cxx.hpp:113,
+ *  sarsolver_cxx.cpp:26-30; bound at sarsolver_cxx.py:214-216).
+ * Overwrites info->phase_history ([P,N], FFT order) with A x where
+ * x = info->scattering_amplitudes. */
+int forward_evaluate(SarCalculationInfo *info);
+
+/* Replaces the reference's `adjoint_evaluate(SarCalculationInfo&)`
+ * (src/sarsolver/cxx_binding/sarsolver_cxx.hpp:114,
+ *  sarsolver_cxx.cpp:32-36; bound at sarsolver_cxx.py:218-220).
+ * Overwrites info->scattering_amplitudes ([M]) with the reference's
+ * backprojection of info->phase_history ([P,N], FFT order). */
+int adjoint_evaluate(SarCalculationInfo *info);
+
+/* Opaque plan: geometry, constants, FFT plans and scratch on one device. */
+typedef struct sarhip_plan_s *sarhip_plan_t;
+
+/* Create a plan on `device`.  Geometry is read from HOST memory (same layouts
+ * as SarCalculationInfo; waveform in FFT order) and uploaded once.  Sizes and
+ * constants follow the reference's SarWorker constructor
+ * (sarsolver_cxx.cpp:219-252): N_w = (size_t)(upsample_ratio * N).
+ * Replaces the per-call SarWorker construction of the reference; it has no
+ * single counterpart there. */
+int sarhip_plan_create(sarhip_plan_t *plan, size_t num_fast_times, size_t num_slow_times,
+                       size_t num_scatterers, const double *transmit_posns,
+                       const double *receive_posns, const double *stab_ref_posns,
+                       const double *scat_posns, const double *waveform_fft,
+                       const double *slow_time_weighting, double centre_frequency,
+                       double sample_frequency, double c_eff, double upsample_ratio,
+                       double sign_multiplier, int interp, int precision, int device);
+
+int sarhip_plan_destroy(sarhip_plan_t plan);
+
+/* Forward projection on device buffers: amps [M] complex64 ->
+ * phase_history [P,N] complex64 in `order`.  Stream-ordered on `stream`
+ * (a hipStream_t, NULL = default stream).  Device counterpart of the
+ * reference's SarWorker::execute_forward_evaluate (sarsolver_cxx.cpp:271-310). */
+int sarhip_forward(sarhip_plan_t plan, const void *amps, void *phase_history, int order,
+                   void *stream);
+
+/* Backprojection on device buffers: phase_history [P,N] complex64 in
+ * `order` -> amps [M] complex64.  Device counterpart of the reference's
+ * SarWorker::execute_adjoint_evaluate (sarsolver_cxx.cpp:319-353). */
+int sarhip_adjoint(sarhip_plan_t plan, const void *phase_history, void *amps, int order,
+                   void *stream);
+
+/* Plan introspection: N_w, bins per metre of bistatic range (by_dx) and the
+ * signed centre wavenumber k_c (sarsolver_cxx.cpp:223-228). */
+int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double *by_dx,
+                     double *centre_wavenumber);
+
+/* Device time (ms) of the most recent forward / adjoint call's kernels, split
+ * by stage: [0] spectral window, [1] FFT, [2] gather/scatter, [3] total.
+ * Timing is recorded only when enabled with sarhip_plan_set_timing(plan, 1). */
+int sarhip_plan_set_timing(sarhip_plan_t plan, int enable);
+int sarhip_plan_last_timing(sarhip_plan_t plan, float *ms4);
+
+/* Message describing the most recent failure on the calling thread. */
+const char *sarhip_last_error(void);
+
+/* Library version string. */
+const char *sarhip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SARSOLVER_HIP_H */

@@ -1,0 +1,2 @@
+from .hypothesis import block_hypothesis_vector, BaseSarScene
+from .measurement import BaseSimpleSarAperture, block_phase_history_array
