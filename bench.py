@@ -1,0 +1,175 @@
+"""Benchmark: SAR backprojection + forward projection throughput on one MI355X.
+
+A step is one backprojection (A^H) and one forward projection (A) of the
+large single-scene workload (1024 x 1024 pixels at 0.10 m, 2048 pulses x 4096
+samples, complex64 phase history, range upsampling 8, linear interpolation,
+monostatic circular path R = 10 km, h = 5 km, 4 degree aperture), with
+device-resident inputs.  The reported value is (2 * M * P) pixel-pulse
+updates per step / step time, in Gpixel-pulse/s.
+
+Also reported:
+  roofline      the dominant kernel (backprojection gather) against the MI355X
+                FP32 vector peak: 39 algorithmic flop per pixel-pulse (linear
+                interpolation) x M x P per launch / mean launch time, the launch
+                time measured with HIP events on the library's stream;
+  cpu_baseline  the FP64 CPU oracle (a restatement of the reference algorithm:
+                per-pulse scatter/gather + full N_w FFT per scene block, threads
+                over scene blocks as the reference's joblib pool) on a bounded
+                sample of the same workload (a pulse subset), scaled per
+                pixel-pulse.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2_large]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+FLOP_PER_PX_PULSE = {"linear": 39.0, "nearest": 32.0, "nearest_ref": 32.0}
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2_large")
+    ap.add_argument("--interp", default="linear")
+    ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--cpu-sample-pulses", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def cpu_baseline(w, ap, pos, x, y, interp, pulses):
+    """Time the FP64 oracle (T = cores - 1 scene-block threads) on the first `pulses`
+    pulses of the workload, forward + adjoint; returns pixel-pulse/s."""
+    import oracle.oracle as O
+    O.build()
+    ncores = len(os.sched_getaffinity(0))
+    threads = max(1, ncores - 1)
+    sl = slice(0, pulses)
+    tx = ap.transmitter_positions[sl]
+    srp = ap.stab_ref_posns[sl]
+    W = np.fft.ifftshift(ap.waveform_fft)
+    wts = ap.slow_time_weighting[sl]
+    args = (ap.centre_frequency, ap.sample_bandwidth)
+    t0 = time.perf_counter()
+    O.forward(tx, tx, srp, pos, x, W, wts, *args, ap.num_freqs, c_eff=ap.c_eff,
+              upsample_ratio=w.upsample, interp=interp, num_threads=threads)
+    O.adjoint(tx, tx, srp, pos, np.fft.ifftshift(y[sl], axes=1), W, wts, *args, c_eff=ap.c_eff,
+              upsample_ratio=w.upsample, interp=interp, num_threads=threads)
+    dt = time.perf_counter() - t0
+    pp = 2.0 * pos.shape[0] * pulses
+    return {"value": pp / dt / 1e9, "unit": "Gpixel-pulse/s", "cores": threads, "kind": "port",
+            "sample": f"{w.name} geometry, first {pulses} of {w.pulses} pulses, all "
+                      f"{pos.shape[0]} pixels, one forward + one adjoint ({interp}, FP64, "
+                      f"own FFT), {dt:.1f} s wall"}
+
+
+def traffic_from_profile(kernel: str):
+    """HBM bytes per launch of `kernel`, measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE and committed under profiles/ (null if not measured)."""
+    f = ROOT / "profiles" / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get(kernel, {}).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU "
+              "sharding is a design in DESIGN.md and is not run here.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from sarsolver_amd import SarPlan, synthetic
+
+    w = synthetic.WORKLOADS[args.workload]
+    rng = np.random.default_rng(w.seed)
+    ap, sc = synthetic.build(w)
+    pos = sc.positions
+    x = synthetic.scene_amplitudes(w, rng).reshape(-1)
+    dev = torch.device("cuda", 0)
+    pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
+                 np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting, ap.centre_frequency,
+                 ap.sample_bandwidth, ap.c_eff, w.upsample, ap.sign_multiplier,
+                 interp=args.interp, precision=args.precision, device=dev)
+    xd = torch.from_numpy(x.astype(np.complex64)).to(dev)
+    yd = pl.forward(xd)
+    y = yd.cpu().numpy()
+    img = torch.empty_like(xd)
+    ph = torch.empty_like(yd)
+    for _ in range(args.warmup):
+        pl.adjoint(yd, out=img)
+        pl.forward(xd, out=ph)
+    torch.cuda.synchronize()
+    pl.set_timing(True)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        pl.adjoint(yd, out=img)
+        pl.forward(xd, out=ph)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = start.elapsed_time(stop) / args.steps
+    t_adj = pl.timing("adjoint")
+    t_fwd = pl.timing("forward")
+    pl.set_timing(False)
+    M, P = pos.shape[0], w.pulses
+    pp = float(M) * P
+    value = 2.0 * pp / (ms_step * 1e-3) / 1e9
+    flops = FLOP_PER_PX_PULSE[args.interp] * pp
+    achieved = flops / (t_adj["kernel"] * 1e-3) / 1e12
+    kernel = "bp_gather_fp64_kernel"
+    line = {
+        "metric": "Gpixel-pulse/s backproj+forward",
+        "value": round(value, 3),
+        "unit": "Gpixel-pulse/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64 (f64 range)",
+        "data": "synthetic",
+        "config": {"workload": w.name, "pixels": f"{w.n}x{w.n}", "spacing_m": w.spacing,
+                   "pulses": w.pulses, "samples": w.samples, "upsample": w.upsample,
+                   "interp": args.interp, "precision": args.precision,
+                   "aperture_deg": w.aperture_deg, "global_batch": 1, "parallelism": "none"},
+        "backprojection_gpixel_pulse_s": round(pp / (t_adj["total"] * 1e-3) / 1e9, 3),
+        "forward_gpixel_pulse_s": round(pp / (t_fwd["total"] * 1e-3) / 1e9, 3),
+        "stage_ms": {"adjoint": {k: round(v, 4) for k, v in t_adj.items()},
+                     "forward": {k: round(v, 4) for k, v in t_fwd.items()}},
+        "roofline": {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                     "traffic": traffic_from_profile(kernel),
+                     "algorithmic_flop_per_px_pulse": FLOP_PER_PX_PULSE[args.interp]},
+        "wall_s_timed": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(w, ap, pos, x, y, args.interp, args.cpu_sample_pulses)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -125,11 +125,16 @@ int sarhip_adjoint(sarhip_plan_t plan, const void *phase_history, void *amps, in
 int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double *by_dx,
                      double *centre_wavenumber);
 
-/* Device time (ms) of the most recent forward / adjoint call's kernels, split
- * by stage: [0] spectral window, [1] FFT, [2] gather/scatter, [3] total.
- * Timing is recorded only when enabled with sarhip_plan_set_timing(plan, 1). */
+/* Stage timing with HIP events recorded on the caller's stream.  Enabling
+ * (or re-enabling) clears the log; while enabled every forward / adjoint call
+ * records its stage boundaries without synchronising.  sarhip_plan_timing
+ * waits for the logged calls and returns the mean device time (ms) per call of
+ * `which` (0 = forward, 1 = adjoint), split by stage: [0] spectral window
+ * (window_pad / truncate), [1] FFT, [2] gather (adjoint) or scatter (forward),
+ * [3] total; *ncalls receives the number of calls averaged.  No counterpart in
+ * the reference (it has no instrumentation). */
 int sarhip_plan_set_timing(sarhip_plan_t plan, int enable);
-int sarhip_plan_last_timing(sarhip_plan_t plan, float *ms4);
+int sarhip_plan_timing(sarhip_plan_t plan, int which, float *ms4, int *ncalls);
 
 /* Message describing the most recent failure on the calling thread. */
 const char *sarhip_last_error(void);

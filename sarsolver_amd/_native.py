@@ -48,7 +48,7 @@ class SarCalculationInfo(ctypes.Structure):
 EXPORTED_SYMBOLS = (
     "forward_evaluate", "adjoint_evaluate", "sarhip_plan_create", "sarhip_plan_destroy",
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_set_timing",
-    "sarhip_plan_last_timing", "sarhip_last_error", "sarhip_version",
+    "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
 )
 
 
@@ -90,8 +90,9 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_plan_info.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
-        L.sarhip_plan_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
-        L.sarhip_plan_last_timing.restype = ctypes.c_int
+        L.sarhip_plan_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_int)]
+        L.sarhip_plan_timing.restype = ctypes.c_int
         L.sarhip_last_error.argtypes = []
         L.sarhip_last_error.restype = ctypes.c_char_p
         L.sarhip_version.argtypes = []
@@ -243,14 +244,19 @@ class SarPlan:
         return out
 
     def set_timing(self, enable: bool) -> None:
+        """Start (and clear) or stop the per-call stage event log."""
         check(load_library().sarhip_plan_set_timing(self._handle, int(bool(enable))),
               "sarhip_plan_set_timing")
 
-    def last_timing(self):
-        """Device ms of the last call: (spectral window, FFT, gather/scatter, total)."""
+    def timing(self, which: str):
+        """Mean device ms per logged call of ``which`` ('forward' or 'adjoint'):
+        dict with window, fft, kernel (gather/scatter), total and calls."""
         buf = (ctypes.c_float * 4)()
-        check(load_library().sarhip_plan_last_timing(self._handle, buf), "sarhip_plan_last_timing")
-        return tuple(float(v) for v in buf)
+        n = ctypes.c_int()
+        check(load_library().sarhip_plan_timing(self._handle, {"forward": 0, "adjoint": 1}[which],
+                                                buf, ctypes.byref(n)), "sarhip_plan_timing")
+        return {"window": buf[0], "fft": buf[1], "kernel": buf[2], "total": buf[3],
+                "calls": n.value}
 
 
 def evaluate_host(info: SarCalculationInfo, forward: bool) -> None:

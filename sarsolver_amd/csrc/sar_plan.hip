@@ -117,9 +117,11 @@ struct sarhip_plan_s {
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
 
+    // Stage timing: one set of 4 events per call while enabled, read back lazily.
     bool timing = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    float last_ms[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool[2];  // [0] forward, [1] adjoint
+    size_t ev_used[2] = {0, 0};
+    hipEvent_t *ev = nullptr;            // the current call's 4 events
 
     ~sarhip_plan_s() {
         DeviceGuard g(device);
@@ -139,8 +141,9 @@ struct sarhip_plan_s {
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
         if (fft_bwd) rocfft_plan_destroy(fft_bwd);
         if (fft_info) rocfft_execution_info_destroy(fft_info);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
+        for (auto &pool : ev_pool)
+            for (auto &e : pool)
+                if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -153,22 +156,26 @@ int run_fft(sarhip_plan_t pl, rocfft_plan fp, void *buf, hipStream_t s) {
     return SARHIP_OK;
 }
 
-int record(sarhip_plan_t pl, int i, hipStream_t s) {
-    if (pl->timing) HIP_TRY(hipEventRecord(pl->ev[i], s));
+constexpr size_t kMaxTimedCalls = 4096;
+
+// Reserve the next event set of `which` (0 forward, 1 adjoint) for this call.
+int begin_timing(sarhip_plan_t pl, int which) {
+    pl->ev = nullptr;
+    if (!pl->timing || pl->ev_used[which] >= kMaxTimedCalls) return SARHIP_OK;
+    auto &pool = pl->ev_pool[which];
+    const size_t base = 4 * pl->ev_used[which];
+    while (pool.size() < base + 4) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    pl->ev = pool.data() + base;
+    pl->ev_used[which] += 1;
     return SARHIP_OK;
 }
 
-int finish_timing(sarhip_plan_t pl) {
-    if (!pl->timing) return SARHIP_OK;
-    HIP_TRY(hipEventSynchronize(pl->ev[3]));
-    float a = 0, b = 0, c = 0;
-    HIP_TRY(hipEventElapsedTime(&a, pl->ev[0], pl->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&b, pl->ev[1], pl->ev[2]));
-    HIP_TRY(hipEventElapsedTime(&c, pl->ev[2], pl->ev[3]));
-    pl->last_ms[0] = a;
-    pl->last_ms[1] = b;
-    pl->last_ms[2] = c;
-    pl->last_ms[3] = a + b + c;
+int record(sarhip_plan_t pl, int i, hipStream_t s) {
+    if (pl->ev) HIP_TRY(hipEventRecord(pl->ev[i], s));
     return SARHIP_OK;
 }
 
@@ -440,7 +447,6 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         FFT_TRY(rocfft_execution_info_set_work_buffer(pl->fft_info, pl->fft_work,
                                                       pl->fft_work_size));
     }
-    for (auto &e : pl->ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipDeviceSynchronize());
 
     owner.p = nullptr;
@@ -464,12 +470,33 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *nw, double *by_dx, double *kc) 
 int sarhip_plan_set_timing(sarhip_plan_t plan, int enable) {
     if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
     plan->timing = enable != 0;
+    plan->ev_used[0] = plan->ev_used[1] = 0;
     return SARHIP_OK;
 }
 
-int sarhip_plan_last_timing(sarhip_plan_t plan, float *ms4) {
-    if (!plan || !ms4) return fail(SARHIP_EINVAL, "NULL argument");
-    for (int i = 0; i < 4; ++i) ms4[i] = plan->last_ms[i];
+int sarhip_plan_timing(sarhip_plan_t plan, int which, float *ms4, int *ncalls) {
+    if (!plan || !ms4 || (which != 0 && which != 1)) return fail(SARHIP_EINVAL, "bad argument");
+    DeviceGuard g(plan->device);
+    double acc[3] = {0, 0, 0};
+    const size_t n = plan->ev_used[which];
+    const hipEvent_t *e = plan->ev_pool[which].data();
+    for (size_t c = 0; c < n; ++c, e += 4) {
+        HIP_TRY(hipEventSynchronize(e[3]));
+        for (int k = 0; k < 3; ++k) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e[k], e[k + 1]));
+            acc[k] += ms;
+        }
+    }
+    const double d = n ? (double)n : 1.0;
+    // forward events bracket (scatter, FFT, truncate); adjoint (window, FFT, gather)
+    const int order_fwd[3] = {2, 1, 0};
+    for (int k = 0; k < 3; ++k) {
+        const int slot = which == 0 ? order_fwd[k] : k;
+        ms4[slot] = (float)(acc[k] / d);
+    }
+    ms4[3] = (float)((acc[0] + acc[1] + acc[2]) / d);
+    if (ncalls) *ncalls = (int)n;
     return SARHIP_OK;
 }
 
@@ -481,6 +508,7 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     DeviceGuard g(pl->device);
     hipStream_t s = (hipStream_t)stream;
     int rc;
+    if ((rc = begin_timing(pl, 0))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
     HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm, s));
     HIP_TRY(launch_fill_zero(pl->d_prof, sizeof(float2) * pl->P * pl->nw, s));
@@ -511,14 +539,6 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     HIP_TRY(launch_truncate(pl->d_prof, pl->d_W, pl->d_wfac_fwd, (int)pl->N, (int)pl->nw,
                             (int)pl->P, order == SARHIP_ORDER_CENTRED, (float2 *)phase_history, s));
     if ((rc = record(pl, 3, s))) return rc;
-    // stage order for forward: [0] bin, [1] FFT, [2] truncate (remapped below)
-    if ((rc = finish_timing(pl))) return rc;
-    if (pl->timing) {
-        const float bin = pl->last_ms[0], fft = pl->last_ms[1], trunc = pl->last_ms[2];
-        pl->last_ms[0] = trunc;
-        pl->last_ms[1] = fft;
-        pl->last_ms[2] = bin;
-    }
     return SARHIP_OK;
 }
 
@@ -530,6 +550,7 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     DeviceGuard g(pl->device);
     hipStream_t s = (hipStream_t)stream;
     int rc;
+    if ((rc = begin_timing(pl, 1))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
     HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
                               (int)pl->N, (int)pl->nw, (int)pl->P,
@@ -555,7 +576,7 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     a.mono = pl->mono;
     HIP_TRY(launch_bp_gather(a, s));
     if ((rc = record(pl, 3, s))) return rc;
-    return finish_timing(pl);
+    return SARHIP_OK;
 }
 
 // Reference-compatible entry points (host FP64 record in, host FP64 out).
