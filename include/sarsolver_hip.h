@@ -125,6 +125,17 @@ int sarhip_adjoint(sarhip_plan_t plan, const void *phase_history, void *amps, in
 int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double *by_dx,
                      double *centre_wavenumber);
 
+/* Plan layout: the precision mode in effect (a series request falls back to
+ * SARHIP_PRECISION_FP64 when the geometry does not bound the series
+ * truncation below 1e-9 bin), whether every valid pulse is monostatic, the
+ * number of valid pulses, the forward LDS window length (bins, 0 = global
+ * atomics), pulses per forward workgroup, the series truncation bound (bins)
+ * and the number of 1024-pixel tiles per forward region.  Any output pointer
+ * may be NULL.  No counterpart in the reference. */
+int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
+                       int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
+                       int *fwd_region_tiles);
+
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
  * (or re-enabling) clears the log; while enabled every forward / adjoint call
  * records its stage boundaries without synchronising.  sarhip_plan_timing

@@ -47,7 +47,8 @@ class SarCalculationInfo(ctypes.Structure):
 # Every symbol declared in include/sarsolver_hip.h.
 EXPORTED_SYMBOLS = (
     "forward_evaluate", "adjoint_evaluate", "sarhip_plan_create", "sarhip_plan_destroy",
-    "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_set_timing",
+    "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
+    "sarhip_plan_set_timing",
     "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
 )
 
@@ -88,6 +89,9 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_adjoint.restype = ctypes.c_int
         L.sarhip_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), dp, dp]
         L.sarhip_plan_info.restype = ctypes.c_int
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip]
+        L.sarhip_plan_layout.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
         L.sarhip_plan_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
@@ -189,6 +193,17 @@ class SarPlan:
         self.working_num_fast_times = nw.value
         self.by_dx = by_dx.value
         self.centre_wavenumber = kc.value
+        vals = [ctypes.c_int() for _ in range(5)]
+        err = ctypes.c_double()
+        reg = ctypes.c_int()
+        check(L.sarhip_plan_layout(handle, *[ctypes.byref(v) for v in vals], ctypes.byref(err),
+                                   ctypes.byref(reg)), "sarhip_plan_layout")
+        inv = {v: k for k, v in PRECISION.items()}
+        self.layout = {"precision": inv[vals[0].value], "monostatic": bool(vals[1].value),
+                       "valid_pulses": vals[2].value, "fwd_window_bins": vals[3].value,
+                       "fwd_pulses_per_group": vals[4].value, "series_err_bins": err.value,
+                       "fwd_region_tiles": reg.value}
+        self.effective_precision = self.layout["precision"]
 
     def close(self) -> None:
         if getattr(self, "_handle", None):

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sarhip {
 
 constexpr int kInterpNearestRef = 0;
@@ -20,15 +22,43 @@ constexpr int kPrecSeries = 1;
 constexpr int kPrecHilo = 2;
 constexpr int kPrecFp32 = 3;
 
-// One record per pulse (64 bytes).  Kernels index it with a wave-uniform pulse
-// index, so the compiler reads it with scalar loads into SGPRs.
+// Pixels per tile: a workgroup of the gather kernel owns one tile (256 threads,
+// kPixPerThread pixels each); the range series is expanded about the tile's
+// centre.  A forward workgroup owns a region of consecutive tiles and a group
+// of pulses (both chosen per plan from the LDS budget).
+constexpr int kThreads = 256;
+constexpr int kPixPerThread = 4;
+constexpr int kTilePixels = kThreads * kPixPerThread;  // 1024
+// |frac(v) - 1/2| below this (in bins) sends a nearest-neighbour decision of
+// the series path to an exact FP64 re-evaluation with the reference's formula.
+constexpr double kTieMargin = 1.0e-7;
+
+// One record per pulse (64 bytes), indexed with a wave-uniform pulse index so
+// the compiler reads it with scalar loads into SGPRs.
 struct PulseConst {
     double tx[3];   // transmitter phase centre
     double rx[3];   // receiver phase centre
     double r_srp;   // bistatic range of the SRP, evaluated as CPP:283
-    double valid;   // 1.0 if the pulse geometry is finite, else 0.0 (pulse skipped)
+    double valid;   // 1.0 if the pulse geometry is finite (invalid pulses are skipped)
 };
 static_assert(sizeof(PulseConst) == 64, "PulseConst must be 64 bytes");
+
+// Series constants of one (tile, pulse).  For a pixel X = X0 + d (X0 the tile
+// centre, d exact) and a leg with phase centre T, a = T - X0, A = |a|:
+//     |T - X| = A sqrt(1 + t),  t = b.d + c |d|^2,  b = -2a/A^2,  c = 1/A^2,
+// and with g(t) = sqrt(1+t) - 1 (5th-order series; the plan bounds the rest):
+//     u     = dR * by_dx   = ku_T g_T + ku_R g_R + cu
+//     theta = dR * kc/2pi  = kt_T g_T + kt_R g_R + ct      (ct reduced mod 1)
+// Monostatic pulses use one leg with the factors doubled (8 doubles).
+struct SeriesMono {
+    double bx, by, bz, c, ku, cu, kt, ct;
+};
+struct SeriesBi {
+    double tbx, tby, tbz, tc, rbx, rby, rbz, rc;
+    double kut, kur, cu, ktt, ktr, ct, pad0, pad1;
+};
+static_assert(sizeof(SeriesMono) == 64, "SeriesMono must be 64 bytes");
+static_assert(sizeof(SeriesBi) == 128, "SeriesBi must be 128 bytes");
 
 // ||a - x|| with the reference's evaluation order: ((dx^2 + dy^2) + dz^2), sqrt.
 __device__ __forceinline__ double ref_distance(const double a0, const double a1, const double a2,
@@ -48,6 +78,16 @@ __device__ __forceinline__ double ref_dr(const PulseConst &c, const double x, co
     return (d1 + d2) - c.r_srp;
 }
 
+// g(t) = sqrt(1 + t) - 1 = t/2 - t^2/8 + t^3/16 - 5 t^4/128 + 7 t^5/256 - O(21 t^6/1024).
+// The plan checks that the truncation term stays below 1e-9 bin and 1e-9
+// revolution for every (tile, pulse) before it selects a series mode.
+__device__ __forceinline__ double sqrt1pm1_series(const double t) {
+    return t * fma(t, fma(t, fma(t, fma(t, 0.02734375, -0.0390625), 0.0625), -0.125), 0.5);
+}
+__device__ __forceinline__ float sqrt1pm1_series_f(const float t) {
+    return t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.02734375f, -0.0390625f), 0.0625f), -0.125f), 0.5f);
+}
+
 // Mathematical modulo of an integer-valued double into [0, nw), equal to the
 // reference's math_modulo((long)n, nw) (CPP:22-24) for |n| < 2^52.
 __device__ __forceinline__ int wrap_bin(const double n, const double nwd, const double inv_nw,
@@ -59,40 +99,21 @@ __device__ __forceinline__ int wrap_bin(const double n, const double nwd, const 
     return r;
 }
 
-// sin(2 pi r), cos(2 pi r) for |r| <= 0.5 revolutions.  Quadrant reduction
-// then minimax-style Taylor polynomials in t = 4r - q, |t| <= 1/2 (quarter
-// turns), i.e. angle |pi t / 2| <= pi / 4.  Max abs error ~2e-8 in FP32.
-__device__ __forceinline__ void sincos_rev(const float r, float *s, float *c) {
-    const float x = 4.0f * r;
-    const float q = rintf(x);
-    const float t = x - q;                     // exact (Sterbenz)
-    const float a = t * 1.57079632679489662f;  // angle in radians, |a| <= pi/4
-    const float a2 = a * a;
-    // sin a = a (1 - a^2/6 + a^4/120 - a^6/5040 + a^8/362880)
-    float sp = fmaf(a2, 2.75573192e-6f, -1.98412698e-4f);
-    sp = fmaf(a2, sp, 8.33333333e-3f);
-    sp = fmaf(a2, sp, -1.66666667e-1f);
-    const float sa = fmaf(a * a2, sp, a);
-    // cos a = 1 - a^2/2 + a^4/24 - a^6/720 + a^8/40320
-    float cp = fmaf(a2, 2.48015873e-5f, -1.38888889e-3f);
-    cp = fmaf(a2, cp, 4.16666667e-2f);
-    cp = fmaf(a2, cp, -0.5f);
-    const float ca = fmaf(a2, cp, 1.0f);
-    const int qi = ((int)q) & 3;
-    // rotate by q quarter turns
-    float so = sa, co = ca;
-    if (qi == 1) { so = ca; co = -sa; }
-    else if (qi == 2) { so = -sa; co = -ca; }
-    else if (qi == 3) { so = -ca; co = sa; }
-    *s = so;
-    *c = co;
-}
-
 // Phasor e^{i 2 pi theta} for an FP64 phase in revolutions: the integer part
-// is removed in FP64, the remaining |r| <= 1/2 handled in FP32.
+// is removed in FP64 and the remaining |r| <= 1/2 goes to the hardware
+// v_sin_f32 / v_cos_f32, which take their argument in revolutions (measured
+// on MI355X over [-1/2, 1/2]: max abs error 1.2e-7, rms 3.5e-8).
 __device__ __forceinline__ void phasor_rev(const double theta, float *s, float *c) {
     const float r = (float)(theta - rint(theta));
-    sincos_rev(r, s, c);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
 }
+
+// Pixel record for scatter kernels (64 bytes, tile order): offset from the
+// series tile centre (exact), |offset|^2, absolute position, finite flag.
+struct PixRec {
+    double dx, dy, dz, d2, x, y, z, ok;
+};
+static_assert(sizeof(PixRec) == 64, "PixRec must be 64 bytes");
 
 }  // namespace sarhip

@@ -6,7 +6,8 @@
 //   window_pad   CPP:331-342  conj(W) * Y into the zero-padded N_w spectrum
 //   (rocFFT)     CPP:344      unnormalised inverse FFT, batched over pulses
 //   bp_gather    CPP:345-350  per pixel: range, bin, profile sample, phase
-//   fp_bin       CPP:284-291  per pulse: scatter a * e^{+i k_c dR} into bins
+//   fp_scatter   CPP:284-291  per pulse: scatter a * e^{+i k_c dR} into bins
+//   fp_reduce    (none)       merge the per-tile LDS windows of fp_scatter
 //   (rocFFT)     CPP:292      unnormalised forward FFT, batched over pulses
 //   truncate     CPP:295-307  keep N modes, multiply by W
 // The slow-time weight w_p (CPP:288, CPP:349) is folded into window_pad /
@@ -22,6 +23,11 @@ namespace sarhip {
 // Spectral stages (HBM-bound elementwise kernels)
 // ---------------------------------------------------------------------------
 
+// Two consecutive profile samples (8-byte aligned) read as one 16-byte load.
+struct alignas(8) Float2x2 {
+    float a, b, c, d;
+};
+
 __device__ __forceinline__ float2 cmul(const float2 a, const float2 b) {
     return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
 }
@@ -29,8 +35,8 @@ __device__ __forceinline__ float2 cmul(const float2 a, const float2 b) {
 // K[p, j] for j in [0, N_w): conj(W[k]) * wfac[p] * Y[p, src(k)] on the N
 // support positions, zero elsewhere.  Support: j < ceil(N/2) -> k = j;
 // j >= N_w - floor(N/2) -> k = j - (N_w - N) (CPP:331-342; odd-N middle mode
-// lands at j = floor(N/2)).  `order` selects FFT-order or centred input; the
-// centred case applies ifftshift on read (adjoint.py:36).
+// lands at j = floor(N/2)).  `centred` input applies ifftshift on read
+// (adjoint.py:36).
 __global__ __launch_bounds__(256) void window_pad_kernel(const float2 *__restrict__ Y,
                                                          const float2 *__restrict__ Wconj,
                                                          const float2 *__restrict__ wfac, int N,
@@ -52,7 +58,7 @@ __global__ __launch_bounds__(256) void window_pad_kernel(const float2 *__restric
     K[(size_t)p * nw + j] = v;
 }
 
-// Y[p, k] = W[k] * wfac[p] * G[p, src(k)] (CPP:295-307); centred output
+// Y[p, k] = W[k] * wfac[p] * G[p, src(k)] (CPP:295-307); `centred` output
 // applies fftshift on write (forward.py:44).
 __global__ __launch_bounds__(256) void truncate_kernel(const float2 *__restrict__ G,
                                                        const float2 *__restrict__ W,
@@ -85,165 +91,535 @@ __global__ __launch_bounds__(256) void gather_perm_kernel(const float2 *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Backprojection gather, FP64 ranges (precision = fp64)
-//
-// One thread per pixel (tile order), a workgroup = 256 consecutive pixels of
-// the Morton order.  The pulse loop is wave-uniform: per-pulse constants come
-// from scalar loads.  Ranges are evaluated exactly as the reference does, so
-// bin indices are bit-identical to it.  Profile samples are gathered from
-// the upsampled range profiles (L1/L2-resident windows).  Phase: FP64
-// revolutions, integer part removed in FP64, FP32 sin/cos.  Accumulation in
-// FP32 over chunks of 64 pulses, chunks summed in FP64.
+// Series constants per (tile, pulse), evaluated once per plan in FP64.
 // ---------------------------------------------------------------------------
-template <int INTERP, bool MONO>
-__global__ __launch_bounds__(256) void bp_gather_fp64_kernel(
-    const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
-    const int *__restrict__ perm, int M, const PulseConst *__restrict__ pc, int P,
-    const float2 *__restrict__ prof, int nw, double by_dx, double kc_rev, double inv_nw,
-    float2 *__restrict__ out) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const bool active = i < M;
-    const int ii = active ? i : M - 1;
-    const double x = px[ii], y = py[ii], z = pz[ii];
-    const bool finite = isfinite(x) && isfinite(y) && isfinite(z);
-    const double nwd = (double)nw;
-    double sr = 0.0, si = 0.0;
-    float ar = 0.f, ai = 0.f;
-    for (int p = 0; p < P; ++p) {
-        const PulseConst c = pc[p];
-        if (c.valid == 0.0) continue;
-        const double dr = ref_dr<MONO>(c, x, y, z);
-        const float2 *row = prof + (size_t)p * nw;
-        const double u = dr * by_dx;
-        float hr, hi;
-        if (INTERP == kInterpLinear) {
-            const double f0 = floor(u);
-            const float fr = (float)(u - f0);
-            const int n0 = wrap_bin(f0, nwd, inv_nw, nw);
-            const int n1 = (n0 + 1 == nw) ? 0 : n0 + 1;
-            const float2 h0 = row[n0], h1 = row[n1];
-            hr = fmaf(fr, h1.x - h0.x, h0.x);
-            hi = fmaf(fr, h1.y - h0.y, h0.y);
-        } else {
-            const double v = (INTERP == kInterpNearestRef) ? 0.5 + u : u;
-            const int n = wrap_bin(round(v), nwd, inv_nw, nw);
-            const float2 h = row[n];
-            hr = h.x;
-            hi = h.y;
-        }
-        float s, co;
-        phasor_rev(dr * kc_rev, &s, &co);
-        // h * e^{-i theta}
-        ar += fmaf(hr, co, hi * s);
-        ai += fmaf(hi, co, -(hr * s));
-        if ((p & 63) == 63) {
-            sr += (double)ar;
-            si += (double)ai;
-            ar = 0.f;
-            ai = 0.f;
-        }
-    }
-    sr += (double)ar;
-    si += (double)ai;
-    if (active) {
-        if (!finite) sr = si = 0.0;
-        out[perm[i]] = make_float2((float)sr, (float)si);
+__device__ __forceinline__ void leg_consts(const double *T, const double4 X0, double &bx,
+                                           double &by, double &bz, double &c, double &A) {
+    const double ax = T[0] - X0.x, ay = T[1] - X0.y, az = T[2] - X0.z;
+    const double A2 = fma(ax, ax, fma(ay, ay, az * az));
+    A = sqrt(A2);
+    c = 1.0 / A2;
+    bx = -2.0 * ax * c;
+    by = -2.0 * ay * c;
+    bz = -2.0 * az * c;
+}
+
+template <bool MONO>
+__global__ __launch_bounds__(256) void series_consts_kernel(const double4 *__restrict__ tile_c,
+                                                            int num_tiles,
+                                                            const PulseConst *__restrict__ pc,
+                                                            const int *__restrict__ rows, int Pv,
+                                                            double by_dx, double kc_rev,
+                                                            void *__restrict__ out) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)num_tiles * Pv) return;
+    const int t = (int)(idx / Pv), j = (int)(idx % Pv);
+    const PulseConst c = pc[rows[j]];
+    const double4 X0 = tile_c[t];
+    if (MONO) {
+        SeriesMono s;
+        double A;
+        leg_consts(c.tx, X0, s.bx, s.by, s.bz, s.c, A);
+        const double C = (A + A) - c.r_srp;  // dR at the tile centre
+        s.ku = 2.0 * A * by_dx;
+        s.cu = C * by_dx;
+        s.kt = 2.0 * A * kc_rev;
+        const double th = C * kc_rev;
+        s.ct = th - rint(th);
+        ((SeriesMono *)out)[idx] = s;
+    } else {
+        SeriesBi s;
+        double At, Ar;
+        leg_consts(c.tx, X0, s.tbx, s.tby, s.tbz, s.tc, At);
+        leg_consts(c.rx, X0, s.rbx, s.rby, s.rbz, s.rc, Ar);
+        const double C = (At + Ar) - c.r_srp;
+        s.kut = At * by_dx;
+        s.kur = Ar * by_dx;
+        s.cu = C * by_dx;
+        s.ktt = At * kc_rev;
+        s.ktr = Ar * kc_rev;
+        const double th = C * kc_rev;
+        s.ct = th - rint(th);
+        s.pad0 = s.pad1 = 0.0;
+        ((SeriesBi *)out)[idx] = s;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Forward binning, FP64 ranges (precision = fp64)
-//
-// Grid (forward tiles) x (pulse groups).  Each workgroup keeps, per pulse of
-// its group, an LDS window of the range profile that covers every bin its
-// tile can reach (tile bounding sphere: |dR - dR_centre| <= 2 radius), adds
-// contributions with LDS float atomics, then flushes each non-zero window
-// bin into the global profile with one float atomic per component.
-// Contributions outside a window (never expected; kept for safety) go
-// straight to global memory.
+// Per-pixel range evaluation for the four precision modes.
 // ---------------------------------------------------------------------------
-template <int INTERP, bool MONO>
-__global__ __launch_bounds__(256) void fp_bin_fp64_kernel(
-    const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
-    const float2 *__restrict__ amps_perm, int M, int tile_len, const double4 *__restrict__ tiles,
-    const PulseConst *__restrict__ pc, int P, int pg, int win, float2 *__restrict__ prof, int nw,
-    double by_dx, double kc_rev, double inv_nw) {
-    extern __shared__ double smem[];
-    double *wbase = smem;                         // [pg] window base bin (unwrapped)
-    float2 *wsum = (float2 *)(smem + pg);         // [pg][win]
-    const int t = blockIdx.x, g = blockIdx.y;
-    const int p0 = g * pg;
-    const int np = min(pg, P - p0);
-    const int m0 = t * tile_len, m1 = min(M, m0 + tile_len);
-    const double nwd = (double)nw;
-    for (int j = threadIdx.x; j < pg * win; j += 256) wsum[j] = make_float2(0.f, 0.f);
-    if ((int)threadIdx.x < np) {
-        const PulseConst c = pc[p0 + threadIdx.x];
-        const double4 tl = tiles[t];
-        const double dr0 = ref_dr<MONO>(c, tl.x, tl.y, tl.z);
-        wbase[threadIdx.x] = floor((dr0 - 2.0 * tl.w) * by_dx) - 2.0;
+struct Pix {
+    double x, y, z;     // absolute position (exact input; tile centre if non-finite)
+    double dx, dy, dz;  // offset from the tile centre (exact)
+    double d2;          // |d|^2
+    bool ok;            // position was finite
+};
+
+__device__ __forceinline__ Pix load_pix(const KernelParams &k, int i, const double4 X0) {
+    Pix p;
+    const bool act = i < k.M;
+    const int ii = act ? i : k.M - 1;
+    double x = k.px[ii], y = k.py[ii], z = k.pz[ii];
+    p.ok = act && isfinite(x) && isfinite(y) && isfinite(z);
+    if (!p.ok) {
+        x = X0.x;
+        y = X0.y;
+        z = X0.z;
     }
-    __syncthreads();
-    for (int m = m0 + (int)threadIdx.x; m < m1; m += 256) {
-        const double x = px[m], y = py[m], z = pz[m];
-        if (!(isfinite(x) && isfinite(y) && isfinite(z))) continue;
-        const float2 a = amps_perm[m];
-        for (int q = 0; q < np; ++q) {
-            const PulseConst c = pc[p0 + q];
-            if (c.valid == 0.0) continue;
-            const double dr = ref_dr<MONO>(c, x, y, z);
-            float s, co;
-            phasor_rev(dr * kc_rev, &s, &co);
-            // e^{+i theta} * a
-            const float cr = fmaf(co, a.x, -(s * a.y));
-            const float ci = fmaf(co, a.y, s * a.x);
-            const double u = dr * by_dx;
-            const double wb = wbase[q];
-            float2 *row = prof + (size_t)(p0 + q) * nw;
-            if (INTERP == kInterpLinear) {
-                const double f0 = floor(u);
-                const float fr = (float)(u - f0);
-                const float gr = 1.0f - fr;
-                const int l = (int)(f0 - wb);
-                if (l >= 0 && l + 1 < win) {
-                    float2 *w0 = wsum + q * win + l;
-                    atomicAdd(&w0[0].x, gr * cr);
-                    atomicAdd(&w0[0].y, gr * ci);
-                    atomicAdd(&w0[1].x, fr * cr);
-                    atomicAdd(&w0[1].y, fr * ci);
+    p.x = x;
+    p.y = y;
+    p.z = z;
+    p.dx = x - X0.x;
+    p.dy = y - X0.y;
+    p.dz = z - X0.z;
+    p.d2 = fma(p.dx, p.dx, fma(p.dy, p.dy, p.dz * p.dz));
+    return p;
+}
+
+// Uniform per-pulse constants: the pulse record and (series / hilo) the
+// (tile, pulse) series record.  All loads are wave-uniform (scalar).
+template <bool MONO>
+struct PulseData {
+    PulseConst pc;
+    typename std::conditional<MONO, SeriesMono, SeriesBi>::type s;
+};
+
+template <int PREC, bool MONO>
+__device__ __forceinline__ void load_pulse(const KernelParams &k, int tile, int j, int p,
+                                           PulseData<MONO> &d) {
+    if constexpr (PREC == kPrecFp64 || PREC == kPrecFp32) {
+        d.pc = k.pc[p];
+    } else {
+        using S = typename std::conditional<MONO, SeriesMono, SeriesBi>::type;
+        d.s = ((const S *)k.series)[(size_t)tile * k.Pv + j];
+    }
+}
+
+// u = dR * by_dx (bins), th = dR * k_c / 2 pi (revolutions).
+template <int PREC, bool MONO>
+__device__ __forceinline__ void eval_range(const KernelParams &k, const PulseData<MONO> &d,
+                                           const Pix &q, double &u, double &th) {
+    if constexpr (PREC == kPrecFp64) {
+        const double dr = ref_dr<MONO>(d.pc, q.x, q.y, q.z);
+        u = dr * k.by_dx;
+        th = dr * k.kc_rev;
+    } else if constexpr (PREC == kPrecSeries) {
+        if constexpr (MONO) {
+            const double t = fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
+            const double g = sqrt1pm1_series(t);
+            u = fma(d.s.ku, g, d.s.cu);
+            th = fma(d.s.kt, g, d.s.ct);
+        } else {
+            const double tt =
+                fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, fma(d.s.tbz, q.dz, d.s.tc * q.d2)));
+            const double tr =
+                fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, fma(d.s.rbz, q.dz, d.s.rc * q.d2)));
+            const double gt = sqrt1pm1_series(tt), gr = sqrt1pm1_series(tr);
+            u = fma(d.s.kut, gt, fma(d.s.kur, gr, d.s.cu));
+            th = fma(d.s.ktt, gt, fma(d.s.ktr, gr, d.s.ct));
+        }
+    } else if constexpr (PREC == kPrecHilo) {
+        // FP64 tile-centre range (cu, ct) + FP32 differential range.
+        const float dx = (float)q.dx, dy = (float)q.dy, dz = (float)q.dz, d2 = (float)q.d2;
+        if constexpr (MONO) {
+            const float t = fmaf((float)d.s.bx, dx,
+                                 fmaf((float)d.s.by, dy, fmaf((float)d.s.bz, dz, (float)d.s.c * d2)));
+            const float g = sqrt1pm1_series_f(t);
+            u = d.s.cu + (double)((float)d.s.ku * g);
+            th = d.s.ct + (double)((float)d.s.kt * g);
+        } else {
+            const float tt = fmaf((float)d.s.tbx, dx,
+                                  fmaf((float)d.s.tby, dy, fmaf((float)d.s.tbz, dz, (float)d.s.tc * d2)));
+            const float tr = fmaf((float)d.s.rbx, dx,
+                                  fmaf((float)d.s.rby, dy, fmaf((float)d.s.rbz, dz, (float)d.s.rc * d2)));
+            const float gt = sqrt1pm1_series_f(tt), gr = sqrt1pm1_series_f(tr);
+            u = d.s.cu + (double)fmaf((float)d.s.kut, gt, (float)d.s.kur * gr);
+            th = d.s.ct + (double)fmaf((float)d.s.ktt, gt, (float)d.s.ktr * gr);
+        }
+    } else {
+        // all-FP32 absolute coordinates
+        const float x = (float)q.x, y = (float)q.y, z = (float)q.z;
+        const float ax = (float)d.pc.tx[0] - x, ay = (float)d.pc.tx[1] - y, az = (float)d.pc.tx[2] - z;
+        const float d1 = sqrtf(fmaf(ax, ax, fmaf(ay, ay, az * az)));
+        float dr;
+        if constexpr (MONO) {
+            dr = 2.0f * d1 - (float)d.pc.r_srp;
+        } else {
+            const float bx = (float)d.pc.rx[0] - x, by = (float)d.pc.rx[1] - y,
+                        bz = (float)d.pc.rx[2] - z;
+            dr = d1 + sqrtf(fmaf(bx, bx, fmaf(by, by, bz * bz))) - (float)d.pc.r_srp;
+        }
+        u = (double)(dr * (float)k.by_dx);
+        th = (double)(dr * (float)k.kc_rev);
+    }
+}
+
+// Nearest-neighbour bin (unwrapped, integer-valued double) with the
+// reference's rule; the series path re-evaluates exactly near a tie.
+template <int INTERP, int PREC, bool MONO>
+__device__ __forceinline__ double nearest_bin(const KernelParams &k, const Pix &q, int p, double &u,
+                                              double &th) {
+    constexpr bool REF = INTERP == kInterpNearestRef;
+    double v = REF ? 0.5 + u : u;
+    if constexpr (PREC == kPrecSeries) {
+        const double fl = floor(v);
+        const double fr = v - fl;
+        if (fabs(fr - 0.5) >= kTieMargin) return fr > 0.5 ? fl + 1.0 : fl;
+        // rare: exact reference arithmetic (CPP:287-289 / CPP:346-347)
+        const PulseConst c = k.pc[p];
+        const double dr = ref_dr<MONO>(c, q.x, q.y, q.z);
+        u = dr * k.by_dx;
+        th = dr * k.kc_rev;
+        v = REF ? 0.5 + u : u;
+    }
+    return round(v);
+}
+
+// Bin index in [0, nw) of an unwrapped integer-valued bin; `span` = 1 for
+// nearest, 2 for linear (so n + 1 stays in range on the fast path).
+__device__ __forceinline__ int bin_index(const KernelParams &k, double n, int span) {
+    int ni = (int)n;
+    if ((unsigned)ni > (unsigned)(k.nw - span)) {
+        ni = wrap_bin(n, k.nwd, k.inv_nw, k.nw);
+        ni = min(max(ni, 0), k.nw - 1);
+    }
+    return ni;
+}
+
+// ---------------------------------------------------------------------------
+// Backprojection gather
+//
+// A workgroup owns one tile of kTilePixels consecutive pixels of the Morton
+// order (kPixPerThread per thread).  The pulse loop is wave-uniform: per-pulse
+// or per-(tile, pulse) constants come through scalar loads.  Profile samples
+// are gathered from the upsampled range profiles (L1/L2-resident windows).
+// Phase in FP64 revolutions, integer part removed, FP32 sin/cos.  Products and
+// sums in FP32; per-pixel sums are folded into FP64 every 64 pulses.
+// ---------------------------------------------------------------------------
+template <int INTERP, bool MONO, int PREC>
+__global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
+    const int tile = blockIdx.x;
+    const double4 X0 = k.tile_c[tile];
+    Pix q[kPixPerThread];
+    float ar[kPixPerThread], ai[kPixPerThread];
+    double sr[kPixPerThread], si[kPixPerThread];
+#pragma unroll
+    for (int m = 0; m < kPixPerThread; ++m) {
+        q[m] = load_pix(k, tile * kTilePixels + m * kThreads + (int)threadIdx.x, X0);
+        ar[m] = ai[m] = 0.f;
+        sr[m] = si[m] = 0.0;
+    }
+    for (int j = 0; j < k.Pv; ++j) {
+        const int p = k.rows[j];
+        PulseData<MONO> d;
+        load_pulse<PREC, MONO>(k, tile, j, p, d);
+        const float2 *row = k.prof + (size_t)p * k.nw;
+#pragma unroll
+        for (int m = 0; m < kPixPerThread; ++m) {
+            double u, th;
+            eval_range<PREC, MONO>(k, d, q[m], u, th);
+            float hr, hi;
+            if constexpr (INTERP == kInterpLinear) {
+                const double fl = floor(u);
+                const float fr = (float)(u - fl);
+                const int n0 = bin_index(k, fl, 2);
+                float2 h0, h1;
+                if (n0 + 1 < k.nw) {
+                    const Float2x2 h = *(const Float2x2 *)(row + n0);
+                    h0 = make_float2(h.a, h.b);
+                    h1 = make_float2(h.c, h.d);
                 } else {
-                    const int n0 = wrap_bin(f0, nwd, inv_nw, nw);
-                    const int n1 = (n0 + 1 == nw) ? 0 : n0 + 1;
-                    atomicAdd(&row[n0].x, gr * cr);
-                    atomicAdd(&row[n0].y, gr * ci);
-                    atomicAdd(&row[n1].x, fr * cr);
-                    atomicAdd(&row[n1].y, fr * ci);
+                    h0 = row[n0];
+                    h1 = row[0];
                 }
+                hr = fmaf(fr, h1.x - h0.x, h0.x);
+                hi = fmaf(fr, h1.y - h0.y, h0.y);
             } else {
-                const double nd = round(u);
-                const int l = (int)(nd - wb);
-                if (l >= 0 && l < win) {
-                    float2 *w0 = wsum + q * win + l;
-                    atomicAdd(&w0->x, cr);
-                    atomicAdd(&w0->y, ci);
+                const double n = nearest_bin<INTERP, PREC, MONO>(k, q[m], p, u, th);
+                const float2 h = row[bin_index(k, n, 1)];
+                hr = h.x;
+                hi = h.y;
+            }
+            float s, c;
+            phasor_rev(th, &s, &c);
+            // h * e^{-i theta}
+            ar[m] += fmaf(hr, c, hi * s);
+            ai[m] += fmaf(hi, c, -(hr * s));
+        }
+        if ((j & 63) == 63) {
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                sr[m] += (double)ar[m];
+                si[m] += (double)ai[m];
+                ar[m] = ai[m] = 0.f;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < kPixPerThread; ++m) {
+        const int i = tile * kTilePixels + m * kThreads + (int)threadIdx.x;
+        if (i < k.M) {
+            const double vr = q[m].ok ? sr[m] + (double)ar[m] : 0.0;
+            const double vi = q[m].ok ? si[m] + (double)ai[m] : 0.0;
+            k.out[k.perm[i]] = make_float2((float)vr, (float)vi);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward scatter
+//
+// Grid (regions) x (groups of G valid pulses).  A region is R consecutive
+// series tiles of the Morton order.  Lane q of the workgroup owns pulse
+// j0 + (q mod G): each pulse has its own LDS window covering every bin the
+// region's pixels can reach for that pulse (bounding sphere, |dR - dR_centre|
+// <= 2 radius, two-bin margins), and the 256/G thread streams walk disjoint
+// pixels of the region (pixel records are broadcast loads).  Lanes of one
+// stream never share a window; different streams only collide on the same
+// bin at the same instant.
+//
+// Window bins accumulate in 32-bit fixed point with LDS integer atomics: on
+// gfx950 ds_add_u32 runs ~40x faster than ds_add_f32.  The scale is
+// 2^30 / sum_region(|Re a| + |Im a|): no bin sum can exceed the region's L1
+// norm (interpolation weights are at most 1), so nothing overflows; FP32
+// products carry ~6e-8 relative error and the rounding to integers adds at
+// most 2^-31 of the norm.  The integer parts of the two linear-interpolation
+// shares sum to the rounded total.  A region whose amplitudes are not finite
+// uses float global atomics.
+//
+// Windows are converted back to float and written whole (zeros included) to
+// `partial` with their base bins and max |component|, then merged per pulse
+// by fp_reduce_kernel: no global atomics on the main path and a result that
+// does not depend on scheduling.  A contribution outside its window (not
+// expected) falls back to a global atomic into the profile.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int fix_rn(const float v) { return (int)rintf(v); }
+
+template <int INTERP, bool MONO, int PREC>
+__global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams k) {
+    extern __shared__ int2 wsum[];  // [G][win] fixed point (re, im)
+    __shared__ float l1_red[kThreads];
+    __shared__ int wmax[kThreads];  // bit patterns of non-negative floats, [G]
+    const int G = k.fwd_pulses, R = k.region_tiles;
+    const int region = blockIdx.x;
+    const int j0 = blockIdx.y * G;
+    const int tid = (int)threadIdx.x;
+    const int q = tid & (G - 1);
+    const int stream = tid / G, nstreams = kThreads / G;
+    const int np = min(G, k.Pv - j0);
+    const bool qok = q < np;
+    const int j = j0 + (qok ? q : np - 1);
+    const int p = k.rows[j];
+    const int win = k.win;
+    const int t0 = region * R;
+    const int t1 = min(k.num_tiles, t0 + R);
+    const int base = t0 * kTilePixels;
+    const int npix = min(R * kTilePixels, k.M - base);
+
+    // region L1 norm -> fixed-point scale
+    float l1 = 0.f;
+    for (int i = tid; i < npix; i += kThreads) {
+        const float2 a = k.amps_perm[base + i];
+        l1 += fabsf(a.x) + fabsf(a.y);
+    }
+    l1_red[tid] = l1;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) l1_red[tid] += l1_red[tid + s];
+        __syncthreads();
+    }
+    l1 = l1_red[0];
+    const bool fixed_ok = isfinite(l1) && win > 0;
+    const float scale = (fixed_ok && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
+    const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
+
+    const double4 rc = k.region_c[region];
+    const double wb = floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - 2.0 * rc.w) * k.by_dx) - 2.0;
+    for (int e = tid; e < G * win; e += kThreads) wsum[e] = make_int2(0, 0);
+    __syncthreads();
+    int2 *ws = wsum + q * win;
+    float2 *row = k.prof + (size_t)p * k.nw;
+    if (!(fixed_ok && scale == 0.f)) {  // skip all-zero regions
+        for (int tile = t0; tile < t1; ++tile) {
+            PulseData<MONO> d;
+            load_pulse<PREC, MONO>(k, tile, j, p, d);
+            const int tb = tile * kTilePixels;
+            const int tn = min(kTilePixels, k.M - tb);
+            for (int i = stream; i < tn; i += nstreams) {
+                const PixRec r = k.pixrec[tb + i];
+                if (r.ok == 0.0) continue;
+                const float2 a = k.amps_perm[tb + i];
+                Pix px;
+                px.x = r.x;
+                px.y = r.y;
+                px.z = r.z;
+                px.dx = r.dx;
+                px.dy = r.dy;
+                px.dz = r.dz;
+                px.d2 = r.d2;
+                px.ok = true;
+                double u, th;
+                eval_range<PREC, MONO>(k, d, px, u, th);
+                float sn, cs;
+                double fl;
+                if constexpr (INTERP == kInterpLinear) {
+                    fl = floor(u);
                 } else {
-                    const int n = wrap_bin(nd, nwd, inv_nw, nw);
-                    atomicAdd(&row[n].x, cr);
-                    atomicAdd(&row[n].y, ci);
+                    // forward bin rule round(u) for both nearest modes (CPP:289)
+                    fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
+                }
+                phasor_rev(th, &sn, &cs);
+                const float cr = fmaf(cs, a.x, -(sn * a.y));
+                const float ci = fmaf(cs, a.y, sn * a.x);
+                const int l = (int)(fl - wb);
+                if constexpr (INTERP == kInterpLinear) {
+                    const float fr = (float)(u - fl);
+                    if (fixed_ok && (unsigned)l < (unsigned)(win - 1)) {
+                        if (qok) {
+                            const float sr = cr * scale, si = ci * scale;
+                            const int tr = fix_rn(sr), ti = fix_rn(si);
+                            const int r1 = fix_rn(fr * sr), i1 = fix_rn(fr * si);
+                            atomicAdd(&ws[l].x, tr - r1);
+                            atomicAdd(&ws[l].y, ti - i1);
+                            atomicAdd(&ws[l + 1].x, r1);
+                            atomicAdd(&ws[l + 1].y, i1);
+                        }
+                    } else if (qok) {
+                        const float gr = 1.0f - fr;
+                        const int n0 = bin_index(k, fl, 1);
+                        const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
+                        atomicAdd(&row[n0].x, gr * cr);
+                        atomicAdd(&row[n0].y, gr * ci);
+                        atomicAdd(&row[n1].x, fr * cr);
+                        atomicAdd(&row[n1].y, fr * ci);
+                    }
+                } else {
+                    if (fixed_ok && (unsigned)l < (unsigned)win) {
+                        if (qok) {
+                            atomicAdd(&ws[l].x, fix_rn(cr * scale));
+                            atomicAdd(&ws[l].y, fix_rn(ci * scale));
+                        }
+                    } else if (qok) {
+                        const int nb = bin_index(k, fl, 1);
+                        atomicAdd(&row[nb].x, cr);
+                        atomicAdd(&row[nb].y, ci);
+                    }
                 }
             }
         }
     }
+    if (tid < G) wmax[tid] = 0;
     __syncthreads();
-    for (int j = threadIdx.x; j < np * win; j += 256) {
-        const int q = j / win, l = j - q * win;
-        const float2 v = wsum[j];
-        if (v.x != 0.f || v.y != 0.f) {
-            const int n = wrap_bin(wbase[q] + (double)l, nwd, inv_nw, nw);
-            float2 *row = prof + (size_t)(p0 + q) * nw;
-            atomicAdd(&row[n].x, v.x);
-            atomicAdd(&row[n].y, v.y);
+    const int NR = k.num_regions;
+    for (int e = tid; e < np * win; e += kThreads) {
+        const int qq = e / win, l = e - qq * win;
+        const int2 v = wsum[qq * win + l];
+        const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
+        k.partial[((size_t)(j0 + qq) * NR + region) * win + l] = f;
+        atomicMax(&wmax[qq], __float_as_int(fmaxf(fabsf(f.x), fabsf(f.y))));
+    }
+    __syncthreads();
+    if (tid < np) {
+        const size_t w = (size_t)(j0 + tid) * NR + region;
+        k.wbases[w] = wb;
+        k.wmax[w] = __int_as_float(wmax[tid]);
+    }
+}
+
+// One workgroup per valid pulse: merge the pulse's window partials into its
+// profile row.  Window bases and maxima are staged in LDS; the covered span of
+// bins is accumulated in LDS chunks while the partials stream through
+// coalesced loads.  The chunk accumulates in 32-bit fixed point with LDS
+// integer atomics, scaled by 2^30 / (sum over windows of the window's max
+// |component|), a bound on any bin sum of the row; if that bound is not
+// finite the chunk accumulates in float instead.  The chunk is then added into
+// the row.
+__global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
+    extern __shared__ int2 acc[];  // [chunk], then [NR] bases (double)
+    __shared__ double red_lo[kThreads], red_hi[kThreads];
+    __shared__ float red_l1[kThreads];
+    const int j = blockIdx.x;
+    const int p = k.rows[j];
+    const int NR = k.num_regions, win = k.win;
+    double *sb = (double *)(acc + chunk);
+    const double *b = k.wbases + (size_t)j * NR;
+    const float *wm = k.wmax + (size_t)j * NR;
+    const float2 *part = k.partial + (size_t)j * NR * win;
+    const int tid = (int)threadIdx.x;
+    double lo = INFINITY, hi = -INFINITY;
+    float l1 = 0.f;
+    for (int t = tid; t < NR; t += kThreads) {
+        const double bt = b[t];
+        sb[t] = bt;
+        lo = fmin(lo, bt);
+        hi = fmax(hi, bt);
+        l1 += wm[t];
+    }
+    red_lo[tid] = lo;
+    red_hi[tid] = hi;
+    red_l1[tid] = l1;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            red_lo[tid] = fmin(red_lo[tid], red_lo[tid + s]);
+            red_hi[tid] = fmax(red_hi[tid], red_hi[tid + s]);
+            red_l1[tid] += red_l1[tid + s];
         }
+        __syncthreads();
+    }
+    lo = red_lo[0];
+    hi = red_hi[0] + (double)win;
+    l1 = red_l1[0];
+    const bool fixed = isfinite(l1);
+    const float scale = (fixed && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
+    const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
+    if (fixed && scale == 0.f) return;  // every window is zero: the row stays zero
+    float2 *facc = (float2 *)acc;
+    float2 *row = k.prof + (size_t)p * k.nw;
+    // thread walks elements e = tid + 256 i of [NR][win] without dividing
+    const int t_step = kThreads / win, l_step = kThreads % win;
+    for (double c0 = lo; c0 < hi; c0 += (double)chunk) {
+        for (int e = tid; e < chunk; e += kThreads) acc[e] = make_int2(0, 0);
+        __syncthreads();
+        int t = tid / win, l = tid % win;
+        const size_t total = (size_t)NR * win;
+        for (size_t e = tid; e < total; e += kThreads) {
+            const float2 v = part[e];
+            const double rel = sb[t] + (double)l - c0;
+            if (rel >= 0.0 && rel < (double)chunk) {
+                const int r = (int)rel;
+                if (fixed) {
+                    atomicAdd(&acc[r].x, (int)rintf(v.x * scale));
+                    atomicAdd(&acc[r].y, (int)rintf(v.y * scale));
+                } else {
+                    atomicAdd(&facc[r].x, v.x);
+                    atomicAdd(&facc[r].y, v.y);
+                }
+            }
+            t += t_step;
+            l += l_step;
+            if (l >= win) {
+                l -= win;
+                t += 1;
+            }
+        }
+        __syncthreads();
+        const int len = (int)fmin((double)chunk, hi - c0);
+        for (int e = tid; e < len; e += kThreads) {
+            const int n = bin_index(k, c0 + (double)e, 1);
+            float2 r = row[n];
+            if (fixed) {
+                const int2 v = acc[e];
+                r.x += (float)v.x * inv_scale;
+                r.y += (float)v.y * inv_scale;
+            } else {
+                r.x += facc[e].x;
+                r.y += facc[e].y;
+            }
+            row[n] = r;
+        }
+        __threadfence_block();
+        __syncthreads();
     }
 }
 
@@ -280,42 +656,71 @@ hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 
     return hipGetLastError();
 }
 
-hipError_t launch_bp_gather(const GatherArgs &a, hipStream_t s) {
-    const int blocks = (a.M + 255) / 256;
-#define SARHIP_BP(I, MO)                                                                      \
-    bp_gather_fp64_kernel<I, MO><<<blocks, 256, 0, s>>>(a.px, a.py, a.pz, a.perm, a.M, a.pc,   \
-                                                        a.P, a.prof, a.nw, a.by_dx, a.kc_rev, \
-                                                        a.inv_nw, a.out)
-    if (a.mono) {
-        if (a.interp == kInterpLinear) SARHIP_BP(kInterpLinear, true);
-        else if (a.interp == kInterpNearest) SARHIP_BP(kInterpNearest, true);
-        else SARHIP_BP(kInterpNearestRef, true);
-    } else {
-        if (a.interp == kInterpLinear) SARHIP_BP(kInterpLinear, false);
-        else if (a.interp == kInterpNearest) SARHIP_BP(kInterpNearest, false);
-        else SARHIP_BP(kInterpNearestRef, false);
-    }
-#undef SARHIP_BP
+hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const PulseConst *pc,
+                                const int *rows, int Pv, double by_dx, double kc_rev, bool mono,
+                                void *out, hipStream_t s) {
+    const size_t n = (size_t)num_tiles * Pv;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (mono)
+        series_consts_kernel<true><<<blocks, 256, 0, s>>>(tile_c, num_tiles, pc, rows, Pv, by_dx,
+                                                           kc_rev, out);
+    else
+        series_consts_kernel<false><<<blocks, 256, 0, s>>>(tile_c, num_tiles, pc, rows, Pv, by_dx,
+                                                            kc_rev, out);
     return hipGetLastError();
 }
 
-hipError_t launch_fp_bin(const BinArgs &a, hipStream_t s) {
-    dim3 grid(a.num_tiles, (a.P + a.pg - 1) / a.pg);
-    const size_t lds = sizeof(double) * a.pg + sizeof(float2) * (size_t)a.pg * a.win;
-#define SARHIP_FP(I, MO)                                                                       \
-    fp_bin_fp64_kernel<I, MO><<<grid, 256, lds, s>>>(a.px, a.py, a.pz, a.amps_perm, a.M,        \
-                                                     a.tile_len, a.tiles, a.pc, a.P, a.pg,      \
-                                                     a.win, a.prof, a.nw, a.by_dx, a.kc_rev,    \
-                                                     a.inv_nw)
-    // nearest_ref and nearest share the forward bin rule round(u) (CPP:289).
-    if (a.mono) {
-        if (a.interp == kInterpLinear) SARHIP_FP(kInterpLinear, true);
-        else SARHIP_FP(kInterpNearest, true);
-    } else {
-        if (a.interp == kInterpLinear) SARHIP_FP(kInterpLinear, false);
-        else SARHIP_FP(kInterpNearest, false);
+template <int I, bool MO>
+static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
+    const dim3 grid(k.num_tiles);
+    switch (k.precision) {
+        case kPrecSeries: bp_gather_kernel<I, MO, kPrecSeries><<<grid, kThreads, 0, s>>>(k); break;
+        case kPrecHilo: bp_gather_kernel<I, MO, kPrecHilo><<<grid, kThreads, 0, s>>>(k); break;
+        case kPrecFp32: bp_gather_kernel<I, MO, kPrecFp32><<<grid, kThreads, 0, s>>>(k); break;
+        default: bp_gather_kernel<I, MO, kPrecFp64><<<grid, kThreads, 0, s>>>(k); break;
     }
-#undef SARHIP_FP
+}
+
+hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s) {
+    if (k.mono) {
+        if (k.interp == kInterpLinear) bp_dispatch_prec<kInterpLinear, true>(k, s);
+        else if (k.interp == kInterpNearest) bp_dispatch_prec<kInterpNearest, true>(k, s);
+        else bp_dispatch_prec<kInterpNearestRef, true>(k, s);
+    } else {
+        if (k.interp == kInterpLinear) bp_dispatch_prec<kInterpLinear, false>(k, s);
+        else if (k.interp == kInterpNearest) bp_dispatch_prec<kInterpNearest, false>(k, s);
+        else bp_dispatch_prec<kInterpNearestRef, false>(k, s);
+    }
+    return hipGetLastError();
+}
+
+template <int I, bool MO>
+static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
+    const dim3 grid(k.num_regions, (k.Pv + k.fwd_pulses - 1) / k.fwd_pulses);
+    const size_t lds = sizeof(int2) * (size_t)k.fwd_pulses * k.win;
+    switch (k.precision) {
+        case kPrecSeries: fp_scatter_kernel<I, MO, kPrecSeries><<<grid, kThreads, lds, s>>>(k); break;
+        case kPrecHilo: fp_scatter_kernel<I, MO, kPrecHilo><<<grid, kThreads, lds, s>>>(k); break;
+        case kPrecFp32: fp_scatter_kernel<I, MO, kPrecFp32><<<grid, kThreads, lds, s>>>(k); break;
+        default: fp_scatter_kernel<I, MO, kPrecFp64><<<grid, kThreads, lds, s>>>(k); break;
+    }
+}
+
+hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s) {
+    // nearest_ref and nearest share the forward bin rule round(u) (CPP:289)
+    if (k.mono) {
+        if (k.interp == kInterpLinear) fp_dispatch_prec<kInterpLinear, true>(k, s);
+        else fp_dispatch_prec<kInterpNearest, true>(k, s);
+    } else {
+        if (k.interp == kInterpLinear) fp_dispatch_prec<kInterpLinear, false>(k, s);
+        else fp_dispatch_prec<kInterpNearest, false>(k, s);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s) {
+    const size_t lds = sizeof(int2) * (size_t)chunk + sizeof(double) * (size_t)k.num_regions;
+    fp_reduce_kernel<<<k.Pv, kThreads, lds, s>>>(k, chunk);
     return hipGetLastError();
 }
 

@@ -7,31 +7,38 @@
 
 namespace sarhip {
 
-struct GatherArgs {
-    const double *px, *py, *pz;  // tile-ordered pixel positions (SoA, FP64)
+// Everything a gather / scatter kernel reads, passed by value.
+struct KernelParams {
+    // pixels, in tile (Morton) order
+    const double *px, *py, *pz;  // FP64 SoA positions
     const int *perm;             // tile order -> caller's pixel index
     int M;
-    const PulseConst *pc;
-    int P;
-    const float2 *prof;  // [P][nw] upsampled range profiles (w_p folded in)
+    int num_tiles;               // ceil(M / kTilePixels)
+    const double4 *tile_c;       // per tile: series centre (multiple of 2^-10 m), radius
+    const PixRec *pixrec;        // [M] per-pixel records (scatter kernels)
+    // pulses
+    const PulseConst *pc;  // [P]
+    const int *rows;       // valid pulse rows [Pv]
+    int Pv;
+    const void *series;  // [num_tiles][Pv] SeriesMono / SeriesBi (series, hilo modes)
+    // profiles [P][nw] complex64
+    float2 *prof;
     int nw;
-    double by_dx, kc_rev, inv_nw;
-    float2 *out;  // [M] caller order
-    int interp;
-    bool mono;
-};
-
-struct BinArgs {
-    const double *px, *py, *pz;
+    double nwd, inv_nw, by_dx, kc_rev;
+    // adjoint output (caller order)
+    float2 *out;
+    // forward
     const float2 *amps_perm;  // [M] tile order
-    int M, tile_len, num_tiles;
-    const double4 *tiles;  // per forward tile: centre xyz, bounding radius
-    const PulseConst *pc;
-    int P, pg, win;
-    float2 *prof;  // [P][nw], zeroed before the launch
-    int nw;
-    double by_dx, kc_rev, inv_nw;
-    int interp;
+    int num_regions;          // forward regions of region_tiles consecutive tiles
+    int region_tiles;
+    const double4 *region_c;  // per region: bounding-sphere centre, radius
+    int fwd_pulses;           // pulses (LDS windows) per forward workgroup, power of 2
+    int win;                  // LDS window length per pulse (0 = no windows)
+    float2 *partial;          // [Pv][num_regions][win]
+    double *wbases;           // [Pv][num_regions]
+    float *wmax;              // [Pv][num_regions] max |component| of each window
+    // variant
+    int interp, precision;
     bool mono;
 };
 
@@ -42,7 +49,11 @@ hipError_t launch_truncate(const float2 *G, const float2 *W, const float2 *wfac,
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s);
 hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 *dst,
                               hipStream_t s);
-hipError_t launch_bp_gather(const GatherArgs &a, hipStream_t s);
-hipError_t launch_fp_bin(const BinArgs &a, hipStream_t s);
+hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const PulseConst *pc,
+                                const int *rows, int Pv, double by_dx, double kc_rev, bool mono,
+                                void *out, hipStream_t s);
+hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
+hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
+hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 
 }  // namespace sarhip

@@ -100,17 +100,26 @@ struct sarhip_plan_s {
     size_t N = 0, P = 0, M = 0, nw = 0;
     double fc = 0, fs = 0, c_eff = 0, U = 0, sgn = 1;
     double by_dx = 0, kc = 0, kc_rev = 0, inv_nw = 0;
-    int interp = 0, precision = 0;
+    int interp = 0, precision = 0, requested_precision = 0;
     bool mono = false;
+    int Pv = 0;                  // valid pulses
+    int num_tiles = 0, win = 0, reduce_chunk = 0;
+    int num_regions = 0, region_tiles = 1, fwd_pulses = 32;
+    double series_err_bins = 0;  // bound on the series truncation term (bins)
 
     PulseConst *d_pc = nullptr;
+    int *d_rows = nullptr;
     double *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr;
     int *d_perm = nullptr;
+    double4 *d_tile_c = nullptr, *d_region_c = nullptr;
+    PixRec *d_pixrec = nullptr;
+    void *d_series = nullptr;
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
-    double4 *d_ftiles = nullptr;
-    int f_tile_len = 0, f_num_tiles = 0, f_pg = 0, f_win = 0;
     float2 *d_prof = nullptr;
     float2 *d_amps_perm = nullptr;
+    float2 *d_partial = nullptr;
+    double *d_wbases = nullptr;
+    float *d_wmax = nullptr;
 
     rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
     rocfft_execution_info fft_info = nullptr;
@@ -126,17 +135,24 @@ struct sarhip_plan_s {
     ~sarhip_plan_s() {
         DeviceGuard g(device);
         dev_free(d_pc);
+        dev_free(d_rows);
         dev_free(d_px);
         dev_free(d_py);
         dev_free(d_pz);
         dev_free(d_perm);
+        dev_free(d_tile_c);
+        dev_free(d_region_c);
+        dev_free(d_pixrec);
+        dev_free(d_series);
         dev_free(d_W);
         dev_free(d_Wconj);
         dev_free(d_wfac_fwd);
         dev_free(d_wfac_adj);
-        dev_free(d_ftiles);
         dev_free(d_prof);
         dev_free(d_amps_perm);
+        dev_free(d_partial);
+        dev_free(d_wbases);
+        dev_free(d_wmax);
         if (fft_work) (void)hipFree(fft_work);
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
         if (fft_bwd) rocfft_plan_destroy(fft_bwd);
@@ -144,6 +160,42 @@ struct sarhip_plan_s {
         for (auto &pool : ev_pool)
             for (auto &e : pool)
                 if (e) (void)hipEventDestroy(e);
+    }
+
+    KernelParams params() const {
+        KernelParams k;
+        k.px = d_px;
+        k.py = d_py;
+        k.pz = d_pz;
+        k.perm = d_perm;
+        k.M = (int)M;
+        k.num_tiles = num_tiles;
+        k.tile_c = d_tile_c;
+        k.pixrec = d_pixrec;
+        k.pc = d_pc;
+        k.rows = d_rows;
+        k.Pv = Pv;
+        k.series = d_series;
+        k.prof = d_prof;
+        k.nw = (int)nw;
+        k.nwd = (double)nw;
+        k.inv_nw = inv_nw;
+        k.by_dx = by_dx;
+        k.kc_rev = kc_rev;
+        k.out = nullptr;
+        k.amps_perm = d_amps_perm;
+        k.num_regions = num_regions;
+        k.region_tiles = region_tiles;
+        k.region_c = d_region_c;
+        k.fwd_pulses = fwd_pulses;
+        k.win = win;
+        k.partial = d_partial;
+        k.wbases = d_wbases;
+        k.wmax = d_wmax;
+        k.interp = interp;
+        k.precision = precision;
+        k.mono = mono;
+        return k;
     }
 };
 
@@ -179,11 +231,13 @@ int record(sarhip_plan_t pl, int i, hipStream_t s) {
     return SARHIP_OK;
 }
 
-// Forward tiling: consecutive runs of `len` pixels in Morton order; centre =
-// bounding-box midpoint, radius = max distance to it (inflated slightly).
+// Tiles: consecutive runs of `len` pixels in Morton order.  Centre = bounding-
+// box midpoint, rounded to a multiple of 2^-10 m when `snap` (so that pixel
+// offsets from it are exact in FP64); radius = max distance to the centre,
+// inflated slightly.  Tiles without a finite pixel get centre 0, radius 0.
 void make_tiles(const std::vector<double> &sx, const std::vector<double> &sy,
-                const std::vector<double> &sz, size_t M, int len, std::vector<double4> &tiles,
-                double &rad_max) {
+                const std::vector<double> &sz, size_t M, size_t len, bool snap,
+                std::vector<double4> &tiles, double &rad_max) {
     const size_t nt = (M + len - 1) / len;
     tiles.assign(nt, double4{0, 0, 0, 0});
     rad_max = 0.0;
@@ -201,18 +255,44 @@ void make_tiles(const std::vector<double> &sx, const std::vector<double> &sy,
             }
         }
         if (!any) continue;
-        const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]),
-                     cz = 0.5 * (lo[2] + hi[2]);
+        double c[3];
+        for (int k = 0; k < 3; ++k) {
+            c[k] = 0.5 * (lo[k] + hi[k]);
+            if (snap) c[k] = std::ldexp(std::nearbyint(std::ldexp(c[k], 10)), -10);
+        }
         double r = 0.0;
         for (size_t i = a; i < b; ++i) {
-            const double dx = sx[i] - cx, dy = sy[i] - cy, dz = sz[i] - cz;
+            const double dx = sx[i] - c[0], dy = sy[i] - c[1], dz = sz[i] - c[2];
             const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
             if (std::isfinite(d)) r = std::max(r, d);
         }
         r = r * (1.0 + 1e-9) + 1e-9;
-        tiles[t] = double4{cx, cy, cz, r};
+        tiles[t] = double4{c[0], c[1], c[2], r};
         rad_max = std::max(rad_max, r);
     }
+}
+
+// Largest truncation term of the 5th-order series, 21/1024 t^6 per leg, in
+// bins, over every (tile, valid pulse); t <= (2 A r + r^2) / A^2.
+double series_error_bound(const std::vector<double4> &tiles, const std::vector<PulseConst> &pc,
+                          const std::vector<int> &rows, bool mono, double by_dx) {
+    double worst = 0.0;
+    for (const double4 &tl : tiles) {
+        const double r = tl.w;
+        for (int p : rows) {
+            double legs = 0.0;
+            for (int leg = 0; leg < (mono ? 1 : 2); ++leg) {
+                const double *T = leg == 0 ? pc[p].tx : pc[p].rx;
+                const double dx = T[0] - tl.x, dy = T[1] - tl.y, dz = T[2] - tl.z;
+                const double A = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (!(A > 2.0 * r)) return INFINITY;
+                const double t = (2.0 * A * r + r * r) / (A * A);
+                legs += (21.0 / 1024.0) * std::pow(t, 6) * A * (mono ? 2.0 : 1.0);
+            }
+            worst = std::max(worst, legs * by_dx);
+        }
+    }
+    return worst;
 }
 
 int env_int(const char *name, int dflt) {
@@ -238,8 +318,8 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     if (!out) return fail(SARHIP_EINVAL, "plan output pointer is NULL");
     *out = nullptr;
     if (interp < 0 || interp > 2) return fail(SARHIP_EINVAL, "interp must be 0, 1 or 2");
-    if (precision != SARHIP_PRECISION_FP64)
-        return fail(SARHIP_EINVAL, "precision mode not available in this build");
+    if (precision < SARHIP_PRECISION_FP64 || precision > SARHIP_PRECISION_FP32)
+        return fail(SARHIP_EINVAL, "precision must be 0 (fp64), 1 (series), 2 (hilo) or 3 (fp32)");
     const size_t N = num_fast_times, P = num_slow_times, M = num_scatterers;
     if (N == 0 || P == 0 || M == 0) return fail(SARHIP_EINVAL, "N, P and M must be positive");
     if (!tx || !rx || !srp || !pos || !waveform_fft || !slow_time_weighting)
@@ -294,8 +374,9 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     pl->kc_rev = sign_multiplier * centre_frequency / c_eff;  // k_c / (2 pi), revolutions per metre
     pl->inv_nw = 1.0 / (double)nw;
 
-    // Per-pulse constants.
+    // Per-pulse constants; pulses with non-finite geometry are skipped.
     std::vector<PulseConst> pc(P);
+    std::vector<int> rows;
     bool mono = true;
     for (size_t p = 0; p < P; ++p) {
         PulseConst &c = pc[p];
@@ -308,9 +389,14 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         }
         c.r_srp = host_distance(c.tx, srp + 3 * p) + host_distance(c.rx, srp + 3 * p);
         c.valid = (fin && std::isfinite(c.r_srp)) ? 1.0 : 0.0;
-        if (std::memcmp(c.tx, c.rx, sizeof(c.tx)) != 0) mono = false;
+        if (c.valid != 0.0) {
+            rows.push_back((int)p);
+            if (std::memcmp(c.tx, c.rx, sizeof(c.tx)) != 0) mono = false;
+        }
     }
     pl->mono = mono;
+    pl->Pv = (int)rows.size();
+    if (rows.empty()) rows.push_back(0);  // keeps device arrays non-empty; Pv stays 0
 
     // Tile (Morton) order of the pixels.
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -352,49 +438,89 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         sz[i] = pos[3 * (size_t)m + 2];
     }
 
-    // Forward tiling: the largest tile whose LDS window still leaves room for
-    // at least 16 pulses per workgroup.
+    // Series tiles (also the forward workgroup tiles) and per-pixel records.
     std::vector<double4> tiles;
-    int tile_len = 256, pg = 1, win = 0;
+    double rad_t = 0.0;
+    make_tiles(sx, sy, sz, M, kTilePixels, true, tiles, rad_t);
+    pl->num_tiles = (int)tiles.size();
+    std::vector<PixRec> recs(M);
+    for (size_t i = 0; i < M; ++i) {
+        const double4 &c = tiles[i / kTilePixels];
+        PixRec &r = recs[i];
+        const bool fin = std::isfinite(sx[i]) && std::isfinite(sy[i]) && std::isfinite(sz[i]);
+        r.x = fin ? sx[i] : c.x;
+        r.y = fin ? sy[i] : c.y;
+        r.z = fin ? sz[i] : c.z;
+        r.dx = r.x - c.x;
+        r.dy = r.y - c.y;
+        r.dz = r.z - c.z;
+        r.d2 = std::fma(r.dx, r.dx, std::fma(r.dy, r.dy, r.dz * r.dz));
+        r.ok = fin ? 1.0 : 0.0;
+    }
+
+    // Series modes need the truncation term far below the tie margin; if the
+    // geometry does not allow it the series mode falls back to exact FP64.
+    pl->requested_precision = precision;
+    pl->precision = precision;
+    if (precision == SARHIP_PRECISION_SERIES || precision == SARHIP_PRECISION_HILO) {
+        pl->series_err_bins = series_error_bound(tiles, pc, rows, mono, pl->by_dx);
+        const double phase_err = pl->series_err_bins * pl->kc_rev / pl->by_dx;
+        if (precision == SARHIP_PRECISION_SERIES &&
+            !(pl->series_err_bins <= 1e-9 && std::fabs(phase_err) <= 1e-9))
+            pl->precision = SARHIP_PRECISION_FP64;
+    }
+    const bool need_series =
+        pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO;
+
+    // Forward regions: R consecutive tiles share LDS windows of G pulses.
+    // Each window covers every bin the region can reach for one pulse.  The
+    // choice minimises the window partials written and read per call
+    // (proportional to win_R / R) under a 40 KiB LDS budget, which keeps four
+    // workgroups per CU resident (measured on MI355X: occupancy matters more
+    // than partial traffic), preferring G = 32 pulses.  Windows that do not
+    // fit at all fall back to global atomics into the profile.
+    std::vector<double4> regions;
     {
-        const int forced = env_int("SARHIP_FP_TILE", 0);
-        const int cands[] = {4096, 2048, 1024, 512, 256};
-        bool chosen = false;
-        for (int len : cands) {
-            if (forced > 0 && len != forced) continue;
+        const size_t budget = (size_t)std::min(std::max(env_int("SARHIP_FP_LDS_KB", 40), 16), 64) * 1024;
+        const int forced_r = env_int("SARHIP_FP_REGION", 0);
+        const int forced_g = env_int("SARHIP_FP_PULSES", 0);
+        double best_cost = INFINITY;
+        int best_r = 1, best_g = 32, best_w = 0;
+        for (int R : {1, 2, 4, 8, 16}) {
+            if (forced_r > 0 && R != forced_r) continue;
+            std::vector<double4> reg;
             double rad = 0.0;
-            make_tiles(sx, sy, sz, M, len, tiles, rad);
+            make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad);
             const double wd = std::ceil(4.0 * rad * pl->by_dx) + 8.0;
-            const int w = wd > 1e7 ? (1 << 24) : (int)wd;
-            const int g = (int)((kLdsBudget - 64) / (8 * (size_t)w + 8));
-            const int need = (int)std::min<size_t>(16, P);
-            if (g >= need || len == 256 || forced > 0) {
-                tile_len = len;
-                win = g >= 1 ? w : 0;
-                pg = g >= 1 ? std::min<int>(g, 64) : 16;
-                chosen = true;
-                break;
+            if (wd > 1e6) continue;
+            const int w = (int)wd;
+            for (int G : {32, 16}) {
+                if (forced_g > 0 && G != forced_g) continue;
+                if ((size_t)G * w * sizeof(int2) > budget) continue;
+                const double cost = (double)w / R * (G == 32 ? 1.0 : 2.0);
+                if (cost < best_cost) {
+                    best_cost = cost;
+                    best_r = R;
+                    best_g = G;
+                    best_w = w;
+                }
+                break;  // the larger G that fits is preferred for this R
             }
         }
-        if (!chosen) {
-            double rad = 0.0;
-            make_tiles(sx, sy, sz, M, 256, tiles, rad);
-            tile_len = 256;
-            win = 0;
-            pg = 16;
+        if (env_int("SARHIP_FP_NO_WINDOWS", 0)) best_cost = INFINITY;
+        if (!std::isfinite(best_cost)) {  // no window fits: global atomics
+            best_r = 1;
+            best_g = 32;
+            best_w = 0;
         }
-        const int forced_pg = env_int("SARHIP_FP_PG", 0);
-        if (forced_pg > 0 && win > 0) {
-            const int gmax = (int)((kLdsBudget - 64) / (8 * (size_t)win + 8));
-            pg = std::max(1, std::min(forced_pg, gmax));
-        }
-        pg = (int)std::min<size_t>((size_t)pg, P);
-        if (pg < 1) pg = 1;
+        double rad = 0.0;
+        make_tiles(sx, sy, sz, M, (size_t)best_r * kTilePixels, false, regions, rad);
+        pl->region_tiles = best_r;
+        pl->fwd_pulses = best_g;
+        pl->win = best_w;
+        pl->num_regions = (int)regions.size();
+        pl->reduce_chunk = (int)std::min<size_t>((40 * 1024) / sizeof(int2), nw);
     }
-    pl->f_tile_len = tile_len;
-    pl->f_num_tiles = (int)tiles.size();
-    pl->f_pg = pg;
-    pl->f_win = win;
 
     // Waveform and weights (complex64).
     std::vector<float2> W(N), Wc(N), wf(P), wa(P);
@@ -416,18 +542,34 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     };
     int rc;
     if ((rc = upload((void **)&pl->d_pc, pc.data(), sizeof(PulseConst) * P))) return rc;
+    if ((rc = upload((void **)&pl->d_rows, rows.data(), sizeof(int) * rows.size()))) return rc;
     if ((rc = upload((void **)&pl->d_px, sx.data(), sizeof(double) * M))) return rc;
     if ((rc = upload((void **)&pl->d_py, sy.data(), sizeof(double) * M))) return rc;
     if ((rc = upload((void **)&pl->d_pz, sz.data(), sizeof(double) * M))) return rc;
     if ((rc = upload((void **)&pl->d_perm, perm.data(), sizeof(int) * M))) return rc;
+    if ((rc = upload((void **)&pl->d_tile_c, tiles.data(), sizeof(double4) * tiles.size())))
+        return rc;
+    if ((rc = upload((void **)&pl->d_pixrec, recs.data(), sizeof(PixRec) * M))) return rc;
+    if ((rc = upload((void **)&pl->d_region_c, regions.data(), sizeof(double4) * regions.size())))
+        return rc;
     if ((rc = upload((void **)&pl->d_W, W.data(), sizeof(float2) * N))) return rc;
     if ((rc = upload((void **)&pl->d_Wconj, Wc.data(), sizeof(float2) * N))) return rc;
     if ((rc = upload((void **)&pl->d_wfac_fwd, wf.data(), sizeof(float2) * P))) return rc;
     if ((rc = upload((void **)&pl->d_wfac_adj, wa.data(), sizeof(float2) * P))) return rc;
-    if ((rc = upload((void **)&pl->d_ftiles, tiles.data(), sizeof(double4) * tiles.size())))
-        return rc;
     HIP_TRY(hipMalloc((void **)&pl->d_prof, sizeof(float2) * P * nw));
     HIP_TRY(hipMalloc((void **)&pl->d_amps_perm, sizeof(float2) * M));
+    const size_t nrows = std::max<size_t>(1, (size_t)pl->Pv);
+    HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
+    HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
+    if (pl->win > 0)
+        HIP_TRY(hipMalloc((void **)&pl->d_partial,
+                          sizeof(float2) * nrows * pl->num_regions * (size_t)pl->win));
+    if (need_series && pl->Pv > 0) {
+        const size_t rec = mono ? sizeof(SeriesMono) : sizeof(SeriesBi);
+        HIP_TRY(hipMalloc(&pl->d_series, rec * pl->num_tiles * (size_t)pl->Pv));
+        HIP_TRY(launch_series_consts(pl->d_tile_c, pl->num_tiles, pl->d_pc, pl->d_rows, pl->Pv,
+                                     pl->by_dx, pl->kc_rev, mono, pl->d_series, nullptr));
+    }
 
     // rocFFT: batched in-place complex64 transforms of length N_w.
     const size_t len = nw;
@@ -464,6 +606,20 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *nw, double *by_dx, double *kc) 
     if (nw) *nw = plan->nw;
     if (by_dx) *by_dx = plan->by_dx;
     if (kc) *kc = plan->kc;
+    return SARHIP_OK;
+}
+
+int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
+                       int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
+                       int *fwd_region_tiles) {
+    if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
+    if (fwd_region_tiles) *fwd_region_tiles = plan->region_tiles;
+    if (precision) *precision = plan->precision;
+    if (monostatic) *monostatic = plan->mono ? 1 : 0;
+    if (valid_pulses) *valid_pulses = plan->Pv;
+    if (fwd_window_bins) *fwd_window_bins = plan->win;
+    if (fwd_pulses_per_group) *fwd_pulses_per_group = plan->fwd_pulses;
+    if (series_err_bins) *series_err_bins = plan->series_err_bins;
     return SARHIP_OK;
 }
 
@@ -510,29 +666,14 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     int rc;
     if ((rc = begin_timing(pl, 0))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
-    HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm, s));
     HIP_TRY(launch_fill_zero(pl->d_prof, sizeof(float2) * pl->P * pl->nw, s));
-    BinArgs a;
-    a.px = pl->d_px;
-    a.py = pl->d_py;
-    a.pz = pl->d_pz;
-    a.amps_perm = pl->d_amps_perm;
-    a.M = (int)pl->M;
-    a.tile_len = pl->f_tile_len;
-    a.num_tiles = pl->f_num_tiles;
-    a.tiles = pl->d_ftiles;
-    a.pc = pl->d_pc;
-    a.P = (int)pl->P;
-    a.pg = pl->f_pg;
-    a.win = pl->f_win;
-    a.prof = pl->d_prof;
-    a.nw = (int)pl->nw;
-    a.by_dx = pl->by_dx;
-    a.kc_rev = pl->kc_rev;
-    a.inv_nw = pl->inv_nw;
-    a.interp = pl->interp;
-    a.mono = pl->mono;
-    HIP_TRY(launch_fp_bin(a, s));
+    if (pl->Pv > 0) {
+        HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm,
+                                   s));
+        const KernelParams k = pl->params();
+        HIP_TRY(launch_fp_bin(k, s));
+        if (pl->win > 0) HIP_TRY(launch_fp_reduce(k, pl->reduce_chunk, s));
+    }
     if ((rc = record(pl, 1, s))) return rc;
     if ((rc = run_fft(pl, pl->fft_fwd, pl->d_prof, s))) return rc;
     if ((rc = record(pl, 2, s))) return rc;
@@ -558,23 +699,9 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     if ((rc = record(pl, 1, s))) return rc;
     if ((rc = run_fft(pl, pl->fft_bwd, pl->d_prof, s))) return rc;
     if ((rc = record(pl, 2, s))) return rc;
-    GatherArgs a;
-    a.px = pl->d_px;
-    a.py = pl->d_py;
-    a.pz = pl->d_pz;
-    a.perm = pl->d_perm;
-    a.M = (int)pl->M;
-    a.pc = pl->d_pc;
-    a.P = (int)pl->P;
-    a.prof = pl->d_prof;
-    a.nw = (int)pl->nw;
-    a.by_dx = pl->by_dx;
-    a.kc_rev = pl->kc_rev;
-    a.inv_nw = pl->inv_nw;
-    a.out = (float2 *)amps;
-    a.interp = pl->interp;
-    a.mono = pl->mono;
-    HIP_TRY(launch_bp_gather(a, s));
+    KernelParams k = pl->params();
+    k.out = (float2 *)amps;
+    HIP_TRY(launch_bp_gather(k, s));
     if ((rc = record(pl, 3, s))) return rc;
     return SARHIP_OK;
 }
