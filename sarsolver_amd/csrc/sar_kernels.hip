@@ -401,8 +401,14 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     __shared__ float l1_red[kThreads];
     __shared__ int wmax[kThreads];  // bit patterns of non-negative floats, [G]
     const int G = k.fwd_pulses, R = k.region_tiles;
-    const int region = blockIdx.x;
-    const int j0 = blockIdx.y * G;
+    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
+    // groups of one region are given ids congruent mod 8 and run back to back
+    // on one XCD, whose L2 then serves the region's pixel records to all of them.
+    const int bid = (int)blockIdx.x, npg = k.fwd_groups;
+    const int slot = bid >> 3;
+    const int region = (bid & 7) + 8 * (slot / npg);
+    const int j0 = (slot % npg) * G;
+    if (region >= k.num_regions) return;
     const int tid = (int)threadIdx.x;
     const int q = tid & (G - 1);
     const int stream = tid / G, nstreams = kThreads / G;
@@ -433,8 +439,28 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     const float scale = (fixed_ok && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
 
+    // Window base: for a leg with phase centre T, unit vector e from the region
+    // centre Xc to T and a pixel X = Xc + d in the region's box of half-extents h,
+    // |T - X| - |T - Xc| lies in [-s, s + r^2 / (2 (|T - Xc| - r))] with
+    // s = sum_i |e_i| h_i and r the region radius, so dR - dR(Xc) >= -(s_T + s_R).
     const double4 rc = k.region_c[region];
-    const double wb = floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - 2.0 * rc.w) * k.by_dx) - 2.0;
+    const double4 rh = k.region_h[region];
+    double sleg;
+    {
+        const PulseConst &c = k.pc[p];
+        const double ax = c.tx[0] - rc.x, ay = c.tx[1] - rc.y, az = c.tx[2] - rc.z;
+        const double inv = 1.0 / sqrt(fma(ax, ax, fma(ay, ay, az * az)));
+        sleg = (fabs(ax) * rh.x + fabs(ay) * rh.y + fabs(az) * rh.z) * inv;
+        if (!MONO) {
+            const double bx = c.rx[0] - rc.x, by = c.rx[1] - rc.y, bz = c.rx[2] - rc.z;
+            const double invr = 1.0 / sqrt(fma(bx, bx, fma(by, by, bz * bz)));
+            sleg += (fabs(bx) * rh.x + fabs(by) * rh.y + fabs(bz) * rh.z) * invr;
+        } else {
+            sleg *= 2.0;
+        }
+    }
+    const double wb =
+        floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - sleg * (1.0 + 1e-9) - 1e-9) * k.by_dx) - 2.0;
     for (int e = tid; e < G * win; e += kThreads) wsum[e] = make_int2(0, 0);
     __syncthreads();
     int2 *ws = wsum + q * win;
@@ -696,7 +722,8 @@ hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s) {
 
 template <int I, bool MO>
 static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
-    const dim3 grid(k.num_regions, (k.Pv + k.fwd_pulses - 1) / k.fwd_pulses);
+    const int nr8 = (k.num_regions + 7) / 8 * 8;
+    const dim3 grid((unsigned)(nr8 * k.fwd_groups));
     const size_t lds = sizeof(int2) * (size_t)k.fwd_pulses * k.win;
     switch (k.precision) {
         case kPrecSeries: fp_scatter_kernel<I, MO, kPrecSeries><<<grid, kThreads, lds, s>>>(k); break;

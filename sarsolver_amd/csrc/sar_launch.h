@@ -31,8 +31,10 @@ struct KernelParams {
     const float2 *amps_perm;  // [M] tile order
     int num_regions;          // forward regions of region_tiles consecutive tiles
     int region_tiles;
-    const double4 *region_c;  // per region: bounding-sphere centre, radius
+    const double4 *region_c;  // per region: bounding-box centre, radius
+    const double4 *region_h;  // per region: bounding-box half-extents (x, y, z)
     int fwd_pulses;           // pulses (LDS windows) per forward workgroup, power of 2
+    int fwd_groups;           // ceil(Pv / fwd_pulses)
     int win;                  // LDS window length per pulse (0 = no windows)
     float2 *partial;          // [Pv][num_regions][win]
     double *wbases;           // [Pv][num_regions]

@@ -104,14 +104,14 @@ struct sarhip_plan_s {
     bool mono = false;
     int Pv = 0;                  // valid pulses
     int num_tiles = 0, win = 0, reduce_chunk = 0;
-    int num_regions = 0, region_tiles = 1, fwd_pulses = 32;
+    int num_regions = 0, region_tiles = 1, fwd_pulses = 32, fwd_groups = 1;
     double series_err_bins = 0;  // bound on the series truncation term (bins)
 
     PulseConst *d_pc = nullptr;
     int *d_rows = nullptr;
     double *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr;
     int *d_perm = nullptr;
-    double4 *d_tile_c = nullptr, *d_region_c = nullptr;
+    double4 *d_tile_c = nullptr, *d_region_c = nullptr, *d_region_h = nullptr;
     PixRec *d_pixrec = nullptr;
     void *d_series = nullptr;
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
@@ -142,6 +142,7 @@ struct sarhip_plan_s {
         dev_free(d_perm);
         dev_free(d_tile_c);
         dev_free(d_region_c);
+        dev_free(d_region_h);
         dev_free(d_pixrec);
         dev_free(d_series);
         dev_free(d_W);
@@ -187,7 +188,9 @@ struct sarhip_plan_s {
         k.num_regions = num_regions;
         k.region_tiles = region_tiles;
         k.region_c = d_region_c;
+        k.region_h = d_region_h;
         k.fwd_pulses = fwd_pulses;
+        k.fwd_groups = fwd_groups;
         k.win = win;
         k.partial = d_partial;
         k.wbases = d_wbases;
@@ -237,9 +240,11 @@ int record(sarhip_plan_t pl, int i, hipStream_t s) {
 // inflated slightly.  Tiles without a finite pixel get centre 0, radius 0.
 void make_tiles(const std::vector<double> &sx, const std::vector<double> &sy,
                 const std::vector<double> &sz, size_t M, size_t len, bool snap,
-                std::vector<double4> &tiles, double &rad_max) {
+                std::vector<double4> &tiles, double &rad_max,
+                std::vector<double4> *half = nullptr) {
     const size_t nt = (M + len - 1) / len;
     tiles.assign(nt, double4{0, 0, 0, 0});
+    if (half) half->assign(nt, double4{0, 0, 0, 0});
     rad_max = 0.0;
     for (size_t t = 0; t < nt; ++t) {
         const size_t a = t * len, b = std::min(M, a + len);
@@ -269,7 +274,47 @@ void make_tiles(const std::vector<double> &sx, const std::vector<double> &sy,
         r = r * (1.0 + 1e-9) + 1e-9;
         tiles[t] = double4{c[0], c[1], c[2], r};
         rad_max = std::max(rad_max, r);
+        if (half) {
+            double h[3];
+            for (int k = 0; k < 3; ++k)
+                h[k] = std::max(hi[k] - c[k], c[k] - lo[k]) * (1.0 + 1e-9) + 1e-9;
+            (*half)[t] = double4{h[0], h[1], h[2], 0.0};
+        }
     }
+}
+
+// Forward LDS window length (bins) for regions with bounding boxes `reg` /
+// half-extents `half`: per leg |T - X| - |T - Xc| lies in
+// [-s, s + r^2 / (2 (A - r))], s = sum_i |e_i| h_i, so the bins of one
+// (region, pulse) span at most (2 S + Q) by_dx plus rounding and the linear
+// neighbour; the window keeps a margin of 8 bins.  Returns INFINITY if a leg
+// is inside a region's bounding sphere.
+double window_bins(const std::vector<double4> &reg, const std::vector<double4> &half,
+                   const std::vector<PulseConst> &pc, const std::vector<int> &rows, bool mono,
+                   double by_dx) {
+    double worst = 0.0;
+    for (size_t t = 0; t < reg.size(); ++t) {
+        const double4 &c = reg[t];
+        const double4 &h = half[t];
+        for (int p : rows) {
+            double S = 0.0, Q = 0.0;
+            for (int leg = 0; leg < 2; ++leg) {
+                const double *T = leg == 0 ? pc[p].tx : pc[p].rx;
+                const double ax = T[0] - c.x, ay = T[1] - c.y, az = T[2] - c.z;
+                const double A = std::sqrt(ax * ax + ay * ay + az * az);
+                if (!(A > c.w)) return INFINITY;
+                S += (std::fabs(ax) * h.x + std::fabs(ay) * h.y + std::fabs(az) * h.z) / A;
+                Q += c.w * c.w / (2.0 * (A - c.w));
+                if (mono) {
+                    S *= 2.0;
+                    Q *= 2.0;
+                    break;
+                }
+            }
+            worst = std::max(worst, (2.0 * S + Q) * by_dx);
+        }
+    }
+    return std::ceil(worst * (1.0 + 1e-9)) + 8.0;
 }
 
 // Largest truncation term of the 5th-order series, 21/1024 t^6 per leg, in
@@ -479,7 +524,7 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     // workgroups per CU resident (measured on MI355X: occupancy matters more
     // than partial traffic), preferring G = 32 pulses.  Windows that do not
     // fit at all fall back to global atomics into the profile.
-    std::vector<double4> regions;
+    std::vector<double4> regions, region_half;
     {
         const size_t budget = (size_t)std::min(std::max(env_int("SARHIP_FP_LDS_KB", 40), 16), 64) * 1024;
         const int forced_r = env_int("SARHIP_FP_REGION", 0);
@@ -488,16 +533,16 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         int best_r = 1, best_g = 32, best_w = 0;
         for (int R : {1, 2, 4, 8, 16}) {
             if (forced_r > 0 && R != forced_r) continue;
-            std::vector<double4> reg;
+            std::vector<double4> reg, hal;
             double rad = 0.0;
-            make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad);
-            const double wd = std::ceil(4.0 * rad * pl->by_dx) + 8.0;
+            make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad, &hal);
+            const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
             if (wd > 1e6) continue;
             const int w = (int)wd;
             for (int G : {32, 16}) {
                 if (forced_g > 0 && G != forced_g) continue;
                 if ((size_t)G * w * sizeof(int2) > budget) continue;
-                const double cost = (double)w / R * (G == 32 ? 1.0 : 2.0);
+                const double cost = (double)w / R * (G == 32 ? 1.0 : 4.0);
                 if (cost < best_cost) {
                     best_cost = cost;
                     best_r = R;
@@ -514,9 +559,11 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
             best_w = 0;
         }
         double rad = 0.0;
-        make_tiles(sx, sy, sz, M, (size_t)best_r * kTilePixels, false, regions, rad);
+        make_tiles(sx, sy, sz, M, (size_t)best_r * kTilePixels, false, regions, rad,
+                   &region_half);
         pl->region_tiles = best_r;
         pl->fwd_pulses = best_g;
+        pl->fwd_groups = (std::max(pl->Pv, 1) + best_g - 1) / best_g;
         pl->win = best_w;
         pl->num_regions = (int)regions.size();
         pl->reduce_chunk = (int)std::min<size_t>((40 * 1024) / sizeof(int2), nw);
@@ -551,6 +598,9 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         return rc;
     if ((rc = upload((void **)&pl->d_pixrec, recs.data(), sizeof(PixRec) * M))) return rc;
     if ((rc = upload((void **)&pl->d_region_c, regions.data(), sizeof(double4) * regions.size())))
+        return rc;
+    if ((rc = upload((void **)&pl->d_region_h, region_half.data(),
+                     sizeof(double4) * region_half.size())))
         return rc;
     if ((rc = upload((void **)&pl->d_W, W.data(), sizeof(float2) * N))) return rc;
     if ((rc = upload((void **)&pl->d_Wconj, Wc.data(), sizeof(float2) * N))) return rc;
