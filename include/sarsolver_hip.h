@@ -153,6 +153,36 @@ const char *sarhip_last_error(void);
 /* Library version string. */
 const char *sarhip_version(void);
 
+/* ---- 3x3 convolutions of the learned CG preconditioner ----------------------
+ * No counterpart in the reference (it has no learned components); these
+ * serve the preconditioner network that accelerates CG on the normal
+ * equations of A.  Activations are NHWC bfloat16 with 32 channels (narrower
+ * layers zero-padded), weights bfloat16, accumulation FP32, zero padding of
+ * one pixel on every side.  Device pointers; `stream` is a hipStream_t (NULL =
+ * default stream).  Return SARHIP_OK, SARHIP_EINVAL or SARHIP_EHIP;
+ * sarhip_conv_last_error() gives the message. */
+
+/* y = act(conv3x3(x, W) + bias): x [B,H,W,32] bf16; wfrag = W[9][32][out_c] in
+ * matrix-core fragment order [tap][nb][lane][8], element (lane l, i) =
+ * W[tap][ci = 8 (l / 16) + i][co = 16 nb + l % 16] (nb < 2 if out_c > 16,
+ * else 1; zero beyond out_c); bias [out_c] f32; y [B,H,W,out_c], FP32 when
+ * out_f32 else bf16; ReLU when relu != 0.  1 <= out_c <= 32. */
+int sarhip_conv3x3_bf16(const void *x, const void *wfrag, const float *bias, int B, int H, int W,
+                        int out_c, int relu, int out_f32, void *y, void *stream);
+
+/* Workspace bytes needed by sarhip_conv3x3_wgrad for these sizes. */
+size_t sarhip_conv3x3_wgrad_workspace(int B, int H, int W, int out_c);
+
+/* Weight and bias gradient of a layer: x [B,H,W,32] bf16 (its input), dy
+ * [B,H,W,co] f32 (gradient of the pre-activation output, co = 32 if out_c > 16
+ * else 16, zero-padded), dwdb = [dW (9*32*co, [tap][ci][co]), db (co)] f32.
+ * Deterministic (per-tile partials summed in a fixed order). */
+int sarhip_conv3x3_wgrad(const void *x, const float *dy, int B, int H, int W, int out_c,
+                         float *dwdb, void *workspace, size_t workspace_bytes, void *stream);
+
+/* Message describing the most recent convolution failure on the calling thread. */
+const char *sarhip_conv_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

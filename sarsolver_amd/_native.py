@@ -50,6 +50,8 @@ EXPORTED_SYMBOLS = (
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
     "sarhip_plan_set_timing",
     "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
+    "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
+    "sarhip_conv_last_error",
 )
 
 

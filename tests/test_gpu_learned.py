@@ -1,0 +1,109 @@
+"""The bf16 3x3 convolution kernels of the learned preconditioner against a
+plain PyTorch FP32 reference, on ragged image sizes (not multiples of the
+16 x 16 tile): forward with bias / ReLU / bf16 or FP32 output, the weight
+gradient, and the layer backward (data gradient through the flipped,
+channel-transposed forward kernel)."""
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+F = torch.nn.functional
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+def _rand_input(dev, B, H, W, cin, seed):
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = torch.zeros(B, H, W, 32)
+    x[..., :cin] = torch.randn(B, H, W, cin, generator=g)
+    return x.to(torch.bfloat16).to(dev)
+
+
+def _torch_conv(x_bf16, w, b):
+    """FP32 reference on the same bf16-rounded operands. w [9, cin, cout]."""
+    cin, cout = w.shape[1], w.shape[2]
+    wr = w.to(torch.bfloat16).float().reshape(3, 3, cin, cout).permute(3, 2, 0, 1)
+    xin = x_bf16.float()[..., :cin].permute(0, 3, 1, 2)
+    return F.conv2d(xin, wr, b, padding=1).permute(0, 2, 3, 1)
+
+
+@pytest.mark.parametrize("cin,cout,relu,out_f32", [
+    (32, 32, True, False), (2, 32, True, False), (32, 2, False, True), (32, 17, True, True),
+    (32, 16, False, False)])
+def test_conv_forward_matches_torch(dev, cin, cout, relu, out_f32):
+    from sarsolver_amd.learned import conv3x3, pack_weights
+    print(f"conv fwd cin={cin} cout={cout} relu={relu} f32={out_f32}", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(cin + cout)
+    w = (torch.randn(9, cin, cout, generator=g) * 0.1).to(dev)
+    b = torch.randn(cout, generator=g).to(dev)
+    x = _rand_input(dev, 2, 37, 50, cin, 3)
+    y = conv3x3(x, pack_weights(w), b, cout, relu, out_f32)
+    ref = _torch_conv(x, w, b)
+    if relu:
+        ref = torch.relu(ref)
+    assert y.shape == (2, 37, 50, cout)
+    err = rel_l2(y.float().cpu().numpy(), ref.cpu().numpy())
+    print(f"  rel L2 {err:.2e}")
+    # FP32 accumulation over 288 products (order differs from torch): ~1e-7;
+    # bf16 output rounding: 2^-9 relative per element at most.
+    assert err <= (1e-5 if out_f32 else 4e-3)
+
+
+@pytest.mark.parametrize("cout", [32, 2])
+def test_conv_weight_gradient_matches_torch(dev, cout):
+    from sarsolver_amd.learned import conv3x3_wgrad
+    print(f"conv wgrad cout={cout}", flush=True)
+    x = _rand_input(dev, 2, 37, 50, 32, 5)
+    co = 32 if cout > 16 else 16
+    g = torch.Generator(device="cpu").manual_seed(11)
+    dy = torch.zeros(2, 37, 50, co)
+    dy[..., :cout] = torch.randn(2, 37, 50, cout, generator=g)
+    dy = dy.to(dev)
+    dw, db = conv3x3_wgrad(x, dy, cout)
+    # reference: autograd of the FP64 conv in the same layout
+    w = torch.zeros(9, 32, cout, dtype=torch.float64, device=dev, requires_grad=True)
+    b = torch.zeros(cout, dtype=torch.float64, device=dev, requires_grad=True)
+    wr = w.reshape(3, 3, 32, cout).permute(3, 2, 0, 1)
+    y = F.conv2d(x.double().permute(0, 3, 1, 2), wr, b, padding=1).permute(0, 2, 3, 1)
+    (y * dy[..., :cout].double()).sum().backward()
+    e1 = rel_l2(dw[:, :, :cout].cpu().numpy(), w.grad.cpu().numpy())
+    e2 = rel_l2(db[:cout].cpu().numpy(), b.grad.cpu().numpy())
+    print(f"  dW rel L2 {e1:.2e}, db rel L2 {e2:.2e}")
+    assert e1 <= 1e-5 and e2 <= 1e-5
+    if cout < co:
+        assert float(dw[:, :, cout:].abs().max()) == 0.0
+
+
+def test_layer_backward_matches_torch(dev):
+    from sarsolver_amd.learned import _ConvLayer
+    print("conv layer backward", flush=True)
+    g = torch.Generator(device="cpu").manual_seed(21)
+    w0 = (torch.randn(9, 32, 32, generator=g) * 0.1).to(dev)
+    b0 = (torch.randn(32, generator=g) * 0.1).to(dev)
+    x = _rand_input(dev, 1, 45, 29, 32, 8)
+    xr = x.clone().requires_grad_(True)
+    w = w0.clone().requires_grad_(True)
+    b = b0.clone().requires_grad_(True)
+    y = _ConvLayer.apply(xr, w, b, True, False, True)
+    gy = torch.randn(y.shape, generator=g).to(dev)
+    (y.float() * gy).sum().backward()
+    # reference: FP32 conv on the bf16-rounded weights the kernel uses
+    wq = w0.to(torch.bfloat16).float().requires_grad_(True)
+    bq = b0.clone().requires_grad_(True)
+    xq = x.float().requires_grad_(True)
+    wr = wq.reshape(3, 3, 32, 32).permute(3, 2, 0, 1)
+    yr = torch.relu(F.conv2d(xq.permute(0, 3, 1, 2), wr, bq, padding=1)).permute(0, 2, 3, 1)
+    (yr * gy).sum().backward()
+    ex = rel_l2(xr.grad.float().cpu().numpy(), xq.grad.cpu().numpy())
+    ew = rel_l2(w.grad.cpu().numpy(), wq.grad.cpu().numpy())
+    eb = rel_l2(b.grad.cpu().numpy(), bq.grad.cpu().numpy())
+    print(f"  dx {ex:.2e}  dW {ew:.2e}  db {eb:.2e}")
+    # the ReLU mask comes from the bf16 output and dx is rounded to bf16
+    assert ex <= 1e-2 and ew <= 1e-2 and eb <= 1e-2

@@ -12,7 +12,7 @@ from conftest import ROOT
 def header_functions():
     text = (ROOT / "include" / "sarsolver_hip.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\**\s*(\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char \*)\s*\**\s*(\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_the_reference_abi():
