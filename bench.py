@@ -16,7 +16,13 @@ Also reported:
                 per-pulse scatter/gather + full N_w FFT per scene block, threads
                 over scene blocks as the reference's joblib pool) on a bounded
                 sample of the same workload (a pulse subset), scaled per
-                pixel-pulse.
+                pixel-pulse;
+  pcg           time to relative residual 1e-3 for (A^H A + 1e-3 I) x = A^H y
+                (A normalised by its 2-norm) on held-out 20 dB scenes of the
+                512^2 workload (1024 pulses x 2048 samples, 3 deg): plain CGLS,
+                PCG with the demodulated circulant base alone, and PCG with the
+                learned preconditioner (base + bf16 CNN, committed weights);
+                medians over the scenes and their time ratios to CGLS.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2_large]
 """
 from __future__ import annotations
@@ -50,6 +56,8 @@ def parse():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy a 512 MiB buffer once before timing (HBM counter scaling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pcg-scenes", type=int, default=64,
+                    help="held-out scenes of the learned-PCG vs CGLS comparison (0: skip)")
     return ap.parse_args()
 
 
@@ -89,6 +97,35 @@ def traffic_from_profile(kernel: str):
         return json.loads(f.read_text()).get(kernel, {}).get("bytes_per_launch")
     except Exception:
         return None
+
+
+def pcg_comparison(dev, scenes: int):
+    """Learned-PCG vs CGLS on the 512^2 workload with the committed weights."""
+    from sarsolver_amd import learned
+    from sarsolver_amd.train_preconditioner import default_weights, workload_operator
+    path = default_weights("c5_scene")
+    if not path.exists():
+        return {"skipped": f"no trained weights at {path.relative_to(ROOT)}"}
+    w, op = workload_operator("c5_scene", dev)
+    n = w.n
+    pre = learned.load_preconditioner(path, op, n)
+    res = learned.evaluate(op, {"circulant": pre.base, "learned": lambda r: pre(r, (n, n))}, n,
+                           scenes=scenes)
+    s = res["solvers"]
+    return {
+        "metric": "time to 1e-3 residual, learned-PCG vs CGLS",
+        "value": round(s["learned"]["time_ratio_vs_cgls"], 4), "unit": "ratio (lower is better)",
+        "workload": f"{w.name}: {n}x{n} px, {w.pulses} pulses x {w.samples} samples, "
+                    f"{w.aperture_deg} deg, lam 1e-3, 20 dB, held-out seed 12345",
+        "scenes": scenes,
+        "median_ms": {k: round(v["median_ms"], 3) for k, v in s.items()},
+        "median_iters": {k: v["median_iters"] for k, v in s.items()},
+        "time_ratio_vs_cgls": {k: round(v["time_ratio_vs_cgls"], 4)
+                               for k, v in s.items() if k != "cgls"},
+        "max_true_residual": {k: float(f"{v['max_true_residual']:.3e}") for k, v in s.items()},
+        "fallbacks": {k: v["fallbacks"] for k, v in s.items()},
+        "tau": pre.base.tau, "weights": str(path.relative_to(ROOT)),
+    }
 
 
 def main():
@@ -176,6 +213,8 @@ def main():
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(w, ap, pos, x, y, args.interp, args.cpu_sample_pulses)
+    if args.pcg_scenes > 0:
+        line["pcg"] = pcg_comparison(dev, args.pcg_scenes)
     print(json.dumps(line), flush=True)
 
 

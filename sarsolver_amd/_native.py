@@ -165,6 +165,8 @@ class SarPlan:
         pos = _f64(scat_posns).reshape(-1)
         W = _c128_interleaved(waveform_fft)
         w = _c128_interleaved(slow_time_weighting)
+        self.geometry = {"tx": tx.reshape(-1, 3), "rx": rx.reshape(-1, 3),
+                         "srp": srp.reshape(-1, 3), "pos": pos.reshape(-1, 3)}
         self.num_slow_times = tx.shape[0] // 3
         self.num_fast_times = W.shape[0] // 2
         self.num_scatterers = pos.shape[0] // 3

@@ -1,39 +1,47 @@
 """Learned preconditioner for CG on the SAR normal equations.
 
 System: (Â^H Â + lam I) x = Â^H y with Â = A / ||A||_2 (the operator normalised
-by its power-iteration norm, so that lam = 1e-3 is a relative shift).
+by its power-iteration norm, so that lam = 1e-3 is a relative shift), solved
+to relative residual 1e-3.
 
-Network: six 3x3 convolutions, 32 channels, 2 (real, imag) channels in and
-out, ReLU between layers; bfloat16 weights and activations, FP32 accumulation,
-on the hand-written gfx950 kernels of csrc/conv_kernels.hip (matrix-core
-forward and data gradient, vector-ALU weight gradient).  FP32 master weights
-are kept for the optimiser.  The preconditioner is
+Preconditioner: M(r) = D [ z + s net(z / s) ],  z = P D^H r,  s = rms(z).
 
-    M(r) = r + s * net(r / s),      s = rms(r),
+* D = diag(exp(i phi)) removes the carrier phase phi of the middle pulse.
+  In these demodulated coordinates the normal operator is close to
+  shift-invariant (in pixel coordinates it is not: the near-field look
+  direction moves the k-space band across the scene).
+* P is the circulant inverse filter 1 / (sigma(k) + tau) of the point-spread
+  function measured through the operator itself, on a zero-padded grid
+  (`CirculantPreconditioner`), with tau chosen on training problems.
+* net: six 3x3 convolutions, 32 channels, 2 (real, imag) channels in and
+  out, ReLU between layers; bfloat16 weights and activations, FP32
+  accumulation, on the hand-written gfx950 kernels of csrc/conv_kernels.hip
+  (matrix-core forward and data gradient, vector-ALU weight gradient); FP32
+  master weights for Adam.  Its last layer starts at zero, so the untrained
+  M is exactly D P D^H; s makes M scale-equivariant.
 
-so it is scale-equivariant (CG residuals shrink by orders of magnitude) and is
-the identity (plain CG) when the last layer is zero.
-
-Training pairs come from the operator itself: random scenes x (point
-scatterers plus Gaussian-smoothed clutter), r = K x with K = Â^H Â + lam I, and
-the loss is the K-energy norm of the error of M as an approximate inverse,
-    L = e^H K e / x^H K x,   e = M(r) - x,
-computed as ||Â e||^2 + lam ||e||^2, so its gradient flows through the
-project's own forward and backprojection kernels (torch_ops.sar_forward).
+Training (`train_unrolled`): backpropagation through `iters` unrolled
+flexible-PCG iterations on fresh random problems, minimising the mean log10
+residual; the operator's gradient flows through the project's own forward
+and backprojection kernels (torch_ops).  Losses that score M one step at a
+time (approximate-inverse error, K-angle of the search direction) improved
+that score while making PCG converge more slowly, because a nonlinear M
+breaks conjugacy between steps; the unrolled loss judges M by the
+convergence it produces.
 """
 from __future__ import annotations
 
 import ctypes
-import json
 import math
 import time
 from dataclasses import dataclass
-from typing import List, Optional
+from pathlib import Path
+from typing import Dict, List, Optional
 
 import numpy as np
 import torch
 
-from ._native import SarPlan, check, load_library
+from ._native import SarPlan, load_library
 from .solvers import cgls, fpcg, true_residual
 from .torch_ops import sar_adjoint, sar_forward
 
@@ -167,7 +175,9 @@ class _ConvLayer(torch.autograd.Function):
 class CnnPreconditioner:
     """M(r) = r + s * net(r / s) on complex images [H, W] or [B, H, W]."""
 
-    def __init__(self, device, layers: int = 6, width: int = C, seed: int = 0):
+    def __init__(self, device, layers: int = 6, width: int = C, seed: int = 0,
+                 base: Optional["CirculantPreconditioner"] = None):
+        self.base = base
         g = torch.Generator(device="cpu").manual_seed(seed)
         chans = [2] + [width] * (layers - 1) + [2]
         self.weights: List[torch.Tensor] = []
@@ -213,12 +223,18 @@ class CnnPreconditioner:
         return torch.complex(x[..., 0], x[..., 1])
 
     def __call__(self, r: torch.Tensor, shape) -> torch.Tensor:
-        """Apply to a flat residual [M] (or [B, M]) of images of `shape` (H, W)."""
+        """Apply to a flat residual [M] (or [B, M]) of images of `shape` (H, W).
+        With a base preconditioner, the network refines its output in the
+        base's demodulated coordinates."""
         flat = r.dim() == 1
-        z = r.reshape((-1,) + tuple(shape))
+        b = self.base
+        zb = r if b is None else b.apply_base(b.to_base(r))
+        z = zb.reshape((-1,) + tuple(shape))
         s = torch.sqrt(torch.mean(torch.abs(z) ** 2, dim=(1, 2), keepdim=True)).clamp_min(1e-30)
-        out = z + s * self.net(z / s)
-        return out.reshape(-1) if flat else out.reshape(r.shape)
+        out = (z + s * self.net(z / s)).reshape(r.shape)
+        if b is not None:
+            out = b.from_base(out)
+        return out.reshape(-1) if flat else out
 
     def state_dict(self):
         return {f"w{i}": w.detach().cpu() for i, w in enumerate(self.weights)} | \
@@ -237,6 +253,7 @@ class NormalisedOperator:
 
     def __init__(self, plan: SarPlan, iters: int = 30, seed: int = 0):
         self.plan = plan
+        self.device = plan.device
         g = torch.Generator(device=plan.device).manual_seed(seed)
         v = torch.randn(plan.num_scatterers, dtype=torch.complex64, device=plan.device,
                         generator=g)
@@ -256,9 +273,112 @@ class NormalisedOperator:
         return sar_adjoint(y, self.plan) * self.scale
 
 
-def random_scenes(side: int, batch: int, gen: "Gens", device) -> torch.Tensor:
-    """[batch, side*side] complex64: point scatterers + Gaussian-smoothed complex clutter
-    with random weights (generated on the device)."""
+def carrier_phase(plan: SarPlan, pulse: Optional[int] = None) -> np.ndarray:
+    """k_c dR_c(x) for every pixel: the phase of the centre-frequency echo of
+    one pulse (default: the middle one), dR_c = |Tx - x| + |Rx - x| minus the
+    same for the stabilisation reference point (FP64, reduced mod 2 pi)."""
+    g = plan.geometry
+    p = plan.num_slow_times // 2 if pulse is None else pulse
+    tx, rx, srp, pos = g["tx"][p], g["rx"][p], g["srp"][p], g["pos"]
+    dr = (np.linalg.norm(pos - tx, axis=1) + np.linalg.norm(pos - rx, axis=1)
+          - np.linalg.norm(srp - tx) - np.linalg.norm(srp - rx))
+    return np.mod(plan.centre_wavenumber * dr, 2.0 * np.pi)
+
+
+class CirculantPreconditioner:
+    """P r = D F^H [ F(D^H r) / (sigma(k) + tau) ] (D^H r zero-padded to
+    `pad` x `pad`): the inverse filter of the operator's point-spread function
+    in demodulated coordinates.
+
+    D = diag(exp(i s phi)) removes the carrier phase phi of the middle pulse
+    (`carrier_phase`); the sign s is the one that moves the PSF's band to
+    zero frequency.  In those coordinates the normal operator D^H K D (same
+    spectrum as K = Â^H Â) is close to shift-invariant, whereas K itself is not
+    (the near-field look direction moves the band across the scene).
+    sigma(k) is the spectrum of D^H K D delta for a delta at the image centre,
+    re-centred at the origin.  P is Hermitian positive definite; tau >= lam
+    regularises the band edges and out-of-band frequencies."""
+
+    def __init__(self, op, side: int, tau: float, pad: Optional[int] = None,
+                 demodulate: bool = True, phase: Optional[np.ndarray] = None):
+        dev = op.device
+        self.side, self.pad, self.tau = side, pad or side, tau
+        n, m = side, self.pad
+        with torch.no_grad():
+            if demodulate:
+                if phase is None:
+                    phase = carrier_phase(op.plan)
+                phi = torch.from_numpy(np.asarray(phase, dtype=np.float64)).to(dev)
+                best = None
+                for sgn in (1.0, -1.0):
+                    d = torch.polar(torch.ones_like(phi), sgn * phi).to(torch.complex64)
+                    h = self._psf(op, d, n)
+                    spec = torch.abs(torch.fft.fft2(h)) ** 2
+                    f = torch.fft.fftfreq(n, device=dev)
+                    # mean squared distance of the band from zero frequency
+                    spread = float((spec * (f[:, None] ** 2 + f[None, :] ** 2)).sum() / spec.sum())
+                    if best is None or spread < best[0]:
+                        best = (spread, d, h)
+                self.demod = best[1]
+                h = best[2]
+            else:
+                self.demod = None
+                h = self._psf(op, None, n)
+            hp = torch.zeros(m, m, dtype=torch.complex64, device=dev)
+            hp[:n, :n] = h
+            hp = torch.roll(hp, shifts=(-(n // 2), -(n // 2)), dims=(0, 1))
+            sig = torch.fft.fft2(hp).real.clamp_min(0.0)
+            self.filter = (1.0 / (sig + tau)).to(torch.float32)
+            self.sigma = sig
+
+    @staticmethod
+    def _psf(op, d, n):
+        delta = torch.zeros(n * n, dtype=torch.complex64, device=op.device)
+        delta[(n // 2) * n + n // 2] = 1.0
+        if d is not None:
+            delta = delta * d
+        h = op.adjoint(op.forward(delta))
+        if d is not None:
+            h = h * d.conj()
+        return h.reshape(n, n)
+
+    def to_base(self, r: torch.Tensor) -> torch.Tensor:
+        return r if self.demod is None else r * self.demod.conj()
+
+    def from_base(self, z: torch.Tensor) -> torch.Tensor:
+        return z if self.demod is None else z * self.demod
+
+    def apply_base(self, z: torch.Tensor) -> torch.Tensor:
+        """Circulant inverse in demodulated coordinates: [..., M] -> [..., M]."""
+        n, m = self.side, self.pad
+        zz = z.reshape((-1, n, n))
+        if m != n:
+            zp = torch.zeros((zz.shape[0], m, m), dtype=zz.dtype, device=zz.device)
+            zp[:, :n, :n] = zz
+        else:
+            zp = zz
+        out = torch.fft.ifft2(torch.fft.fft2(zp) * self.filter)[:, :n, :n]
+        return out.reshape(z.shape).to(z.dtype)
+
+    def __call__(self, r: torch.Tensor, shape=None) -> torch.Tensor:
+        return self.from_base(self.apply_base(self.to_base(r)))
+
+
+@dataclass
+class Gens:
+    cpu_gen: torch.Generator
+    dev_gen: torch.Generator
+
+
+def make_gens(seed: int, device) -> Gens:
+    return Gens(torch.Generator(device="cpu").manual_seed(seed),
+                torch.Generator(device=device).manual_seed(seed))
+
+
+def random_scenes(side: int, batch: int, gen: Gens, device) -> torch.Tensor:
+    """[batch, side*side] complex64: 10-200 point scatterers (complex Gaussian
+    amplitudes, std 3) plus complex Gaussian clutter smoothed by a Gaussian of
+    1-4 px and weighted by U(0, 1), generated on the device."""
     out = []
     k = torch.fft.fftfreq(side, device=device)
     for _ in range(batch):
@@ -279,88 +399,8 @@ def random_scenes(side: int, batch: int, gen: "Gens", device) -> torch.Tensor:
     return torch.stack(out)
 
 
-@dataclass
-class Gens:
-    cpu_gen: torch.Generator
-    dev_gen: torch.Generator
-
-
-def make_gens(seed: int, device) -> Gens:
-    return Gens(torch.Generator(device="cpu").manual_seed(seed),
-                torch.Generator(device=device).manual_seed(seed))
-
-
-def train(op: NormalisedOperator, side: int, lam: float = 1e-3, steps: int = 2000,
-          batch: int = 4, lr: float = 1e-3, seed: int = 1, log_every: int = 100,
-          pre: Optional[CnnPreconditioner] = None, progress=None):
-    """Train M on `steps * batch` random scenes (each used once)."""
-    dev = op.plan.device
-    pre = pre or CnnPreconditioner(dev, seed=seed)
-    opt = torch.optim.Adam(pre.parameters(), lr=lr)
-    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
-    gens = make_gens(seed, dev)
-    history = []
-    t0 = time.perf_counter()
-    for step in range(steps):
-        x = random_scenes(side, batch, gens, dev)
-        with torch.no_grad():
-            r = torch.stack([op.adjoint(op.forward(x[b])) + lam * x[b] for b in range(batch)])
-            den = torch.stack([torch.vdot(x[b], r[b]).real for b in range(batch)]).clamp_min(1e-30)
-        e = pre(r, (side, side)) - x
-        loss = 0.0
-        for b in range(batch):
-            Ae = op.forward(e[b])
-            num = torch.sum(torch.abs(Ae) ** 2) + lam * torch.sum(torch.abs(e[b]) ** 2)
-            loss = loss + num / den[b]
-        loss = loss / batch
-        opt.zero_grad()
-        loss.backward()
-        opt.step()
-        sched.step()
-        if step % log_every == 0 or step == steps - 1:
-            lv = float(loss.detach())
-            history.append((step, lv))
-            if progress is not None:
-                progress(step, lv, time.perf_counter() - t0)
-    torch.cuda.synchronize(dev)
-    return pre, {"history": history, "seconds": time.perf_counter() - t0,
-                 "scenes": steps * batch}
-
-
-def _kapply(op, v, lam):
-    return op.adjoint(op.forward(v)) + lam * v
-
-
-def pcg_iterates(op, c: torch.Tensor, lam: float, precond=None, iters: int = 30,
-                 tol: float = 0.0):
-    """Iterates x_0 = 0, x_1, ... of (flexible, Polak-Ribiere) PCG on K x = c with
-    K = Â^H Â + lam I; plain CG when precond is None.  Stops after `iters`
-    steps or at relative residual <= tol.  Returns (iterates, final x)."""
-    x = torch.zeros_like(c)
-    r = c.clone()
-    z = r if precond is None else precond(r)
-    p = z.clone()
-    rz = torch.vdot(r, z)
-    cn = float(torch.linalg.vector_norm(c))
-    xs = [x]
-    for _ in range(iters):
-        Kp = _kapply(op, p, lam)
-        alpha = rz / torch.vdot(p, Kp)
-        x = x + alpha * p
-        r_new = r - alpha * Kp
-        xs.append(x)
-        if tol and float(torch.linalg.vector_norm(r_new)) <= tol * cn:
-            break
-        z_new = r_new if precond is None else precond(r_new)
-        beta = torch.vdot(r_new - r, z_new) / rz
-        rz = torch.vdot(r_new, z_new)
-        p = z_new + beta * p
-        r = r_new
-    return xs, x
-
-
 def noisy_problem(op, x_true: torch.Tensor, snr_db: float, gen: torch.Generator):
-    """c = Â^H y with y = Â x_true + complex Gaussian noise at `snr_db`."""
+    """y = Â x_true + complex Gaussian noise at `snr_db` (per-sample power ratio)."""
     y = op.forward(x_true)
     noise = torch.complex(torch.randn(y.shape, device=y.device, generator=gen),
                           torch.randn(y.shape, device=y.device, generator=gen))
@@ -368,54 +408,55 @@ def noisy_problem(op, x_true: torch.Tensor, snr_db: float, gen: torch.Generator)
     return (y + noise.to(torch.complex64)).contiguous()
 
 
-def trajectory_pairs(op, side: int, scenes: int, lam: float = 1e-3, snr_db: float = 20.0,
-                     iters: int = 30, seed: int = 7, precond=None):
-    """Training pairs from solver trajectories: for each random scene, the exact
-    solution x* of K x = Â^H y (CG to 1e-5), then for every iterate x_k of a
-    solve (plain CG, or PCG with `precond`) the error e_k = x* - x_k and its
-    residual r_k = K e_k.  Returns (R, E) stacked [n, side*side] complex64."""
-    dev = op.plan.device
-    gens = make_gens(seed, dev)
-    R, E = [], []
+def problem_batch(op, side: int, count: int, gens: Gens, snr_db: float = 20.0) -> torch.Tensor:
+    """[count, M] right-hand sides c = Â^H y of random noisy scenes."""
     with torch.no_grad():
-        for _ in range(scenes):
-            x_true = random_scenes(side, 1, gens, dev)[0]
-            c = op.adjoint(noisy_problem(op, x_true, snr_db, gens.dev_gen))
-            _, x_star = pcg_iterates(op, c, lam, None, iters=400, tol=1e-5)
-            xs, _ = pcg_iterates(op, c, lam, precond, iters=iters, tol=1e-4)
-            for xk in xs:
-                e = x_star - xk
-                R.append(_kapply(op, e, lam))
-                E.append(e)
-    return torch.stack(R), torch.stack(E)
+        xs = random_scenes(side, count, gens, op.device)
+        return torch.stack([op.adjoint(noisy_problem(op, xs[b], snr_db, gens.dev_gen))
+                            for b in range(count)])
 
 
-def train_pairs(op: NormalisedOperator, pre: CnnPreconditioner, R: torch.Tensor,
-                E: torch.Tensor, side: int, lam: float = 1e-3, steps: int = 1000,
-                batch: int = 8, lr: float = 1e-3, seed: int = 3, log_every: int = 100,
-                progress=None):
-    """Minimise the K-energy error of M as an approximate inverse on (r, e) pairs:
-    mean over the batch of (||Â d||^2 + lam ||d||^2) / (e^H r), d = M(r) - e.
-    Adam on FP32 master weights, cosine learning-rate decay."""
-    dev = op.plan.device
+def unrolled_pcg_loss(op, precond, c: torch.Tensor, lam: float, iters: int) -> torch.Tensor:
+    """Mean over t = 1..iters of log10(||r_t|| / ||c||) for the flexible PCG
+    recursion (Polak-Ribiere, as solvers.fpcg) on K x = c, K = Â^H Â + lam I,
+    differentiable end to end.  c: [B, M]; returns the batch mean."""
+    cn = torch.linalg.vector_norm(c, dim=1)
+    r = c
+    z = precond(r)
+    p = z
+    rz = torch.sum(r.conj() * z, dim=1)
+    total = 0.0
+    for _ in range(iters):
+        Kp = torch.stack([op.adjoint(op.forward(p[b])) for b in range(p.shape[0])]) + lam * p
+        alpha = rz / torch.sum(p.conj() * Kp, dim=1)
+        r_new = r - alpha[:, None] * Kp
+        total = total + torch.log10(torch.linalg.vector_norm(r_new, dim=1) / cn)
+        z_new = precond(r_new)
+        beta = torch.sum((r_new - r).conj() * z_new, dim=1) / rz
+        rz = torch.sum(r_new.conj() * z_new, dim=1)
+        p = z_new + beta[:, None] * p
+        r = r_new
+    return torch.mean(total) / iters
+
+
+def train_unrolled(op, pre: CnnPreconditioner, side: int, lam: float = 1e-3, iters: int = 12,
+                   steps: int = 200, batch: int = 2, lr: float = 1e-4, seed: int = 5,
+                   clip: float = 1.0, log_every: int = 25, progress=None):
+    """Train M by backpropagating through `iters` unrolled PCG iterations on
+    fresh random problems (loss `unrolled_pcg_loss`), Adam with cosine decay
+    and gradient-norm clipping."""
     opt = torch.optim.Adam(pre.parameters(), lr=lr)
     sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    den_all = torch.stack([torch.vdot(E[i], R[i]).real for i in range(E.shape[0])])
+    gens = make_gens(seed, op.device)
+    shape = (side, side)
     history = []
     t0 = time.perf_counter()
     for step in range(steps):
-        idx = torch.randint(0, R.shape[0], (batch,), generator=g).to(dev)
-        r, e, den = R[idx], E[idx], den_all[idx].clamp_min(1e-30)
-        d = pre(r, (side, side)) - e
-        loss = 0.0
-        for b in range(batch):
-            Ad = op.forward(d[b])
-            loss = loss + (torch.sum(torch.abs(Ad) ** 2)
-                           + lam * torch.sum(torch.abs(d[b]) ** 2)) / den[b]
-        loss = loss / batch
+        c = problem_batch(op, side, batch, gens)
+        loss = unrolled_pcg_loss(op, lambda r: pre(r, shape), c, lam, iters)
         opt.zero_grad()
         loss.backward()
+        torch.nn.utils.clip_grad_norm_(pre.parameters(), clip)
         opt.step()
         sched.step()
         if step % log_every == 0 or step == steps - 1:
@@ -423,40 +464,83 @@ def train_pairs(op: NormalisedOperator, pre: CnnPreconditioner, R: torch.Tensor,
             history.append((step, lv))
             if progress is not None:
                 progress(step, lv, time.perf_counter() - t0)
-    torch.cuda.synchronize(dev)
+    if op.device.type == "cuda":
+        torch.cuda.synchronize(op.device)
     return pre, {"history": history, "seconds": time.perf_counter() - t0}
 
 
-def evaluate(op: NormalisedOperator, pre: CnnPreconditioner, side: int, scenes: int = 64,
+def choose_tau(op, side: int, taus, problems, lam: float = 1e-3, tol: float = 1e-3,
+               pad: Optional[int] = None, max_iter: int = 300):
+    """The tau of `CirculantPreconditioner` with the fewest median PCG
+    iterations on `problems` (a list of y); returns (tau, {tau: median})."""
+    scores = {}
+    with torch.no_grad():
+        for tau in taus:
+            P = CirculantPreconditioner(op, side, tau, pad=pad)
+            its = [fpcg(op, y, P, lam=lam, tol=tol, max_iter=max_iter).iterations
+                   for y in problems]
+            scores[tau] = float(np.median(its))
+    best = min(scores, key=lambda t: (scores[t], -t))
+    return best, scores
+
+
+def evaluate(op, preconds: Dict[str, object], side: int, scenes: int = 64,
              lam: float = 1e-3, tol: float = 1e-3, snr_db: float = 20.0, seed: int = 12345,
              max_iter: int = 300):
-    """Solve (Â^H Â + lam I) x = Â^H y on held-out scenes with plain CGLS and with
-    the learned flexible PCG; every result is judged by the true residual."""
-    dev = op.plan.device
+    """Solve (Â^H Â + lam I) x = Â^H y on `scenes` held-out noisy scenes with
+    plain CGLS and with PCG under each named preconditioner (callables on
+    flat residuals).  Every solve is scored by its recomputed true residual.
+    Returns medians of iterations and wall time (ms) per solver, the ratio of
+    each PCG's median time to CGLS's, and the per-scene rows."""
+    dev = op.device
     gens = make_gens(seed, dev)
     rows = []
-    shape = (side, side)
     with torch.no_grad():
-        for i in range(scenes):
+        for _ in range(scenes):
             x = random_scenes(side, 1, gens, dev)[0]
-            y = op.forward(x)
-            noise = torch.complex(torch.randn(y.shape, device=dev, generator=gens.dev_gen),
-                                  torch.randn(y.shape, device=dev, generator=gens.dev_gen))
-            noise = noise * torch.sqrt(torch.mean(torch.abs(y) ** 2) / 2 / 10 ** (snr_db / 10))
-            y = (y + noise.to(torch.complex64)).contiguous()
-            torch.cuda.synchronize(dev)
+            y = noisy_problem(op, x, snr_db, gens.dev_gen)
+            if dev.type == "cuda":
+                torch.cuda.synchronize(dev)
             rc = cgls(op, y, lam=lam, tol=tol, max_iter=max_iter)
-            rp = fpcg(op, y, lambda r: pre(r, shape), lam=lam, tol=tol, max_iter=max_iter)
-            rows.append({"cgls_iters": rc.iterations, "cgls_s": rc.seconds,
-                         "cgls_true": true_residual(op, rc.x, y, lam),
-                         "pcg_iters": rp.iterations, "pcg_s": rp.seconds,
-                         "pcg_true": true_residual(op, rp.x, y, lam),
-                         "pcg_fallback": rp.fallback})
-    med = lambda k: float(np.median([r[k] for r in rows]))
-    return {"scenes": scenes, "median_cgls_iters": med("cgls_iters"),
-            "median_pcg_iters": med("pcg_iters"), "median_cgls_ms": med("cgls_s") * 1e3,
-            "median_pcg_ms": med("pcg_s") * 1e3,
-            "time_ratio_pcg_over_cgls": med("pcg_s") / med("cgls_s"),
-            "max_true_residual_cgls": max(r["cgls_true"] for r in rows),
-            "max_true_residual_pcg": max(r["pcg_true"] for r in rows),
-            "fallbacks": sum(r["pcg_fallback"] is not None for r in rows), "rows": rows}
+            row = {"cgls": {"iters": rc.iterations, "ms": rc.seconds * 1e3,
+                            "true_residual": true_residual(op, rc.x, y, lam),
+                            "fallback": None}}
+            for name, M in preconds.items():
+                rp = fpcg(op, y, M, lam=lam, tol=tol, max_iter=max_iter)
+                row[name] = {"iters": rp.iterations, "ms": rp.seconds * 1e3,
+                             "true_residual": true_residual(op, rp.x, y, lam),
+                             "fallback": rp.fallback}
+            rows.append(row)
+    out = {"scenes": scenes, "tol": tol, "lam": lam, "snr_db": snr_db, "solvers": {}}
+    for name in ["cgls", *preconds]:
+        its = [r[name]["iters"] for r in rows]
+        ms = [r[name]["ms"] for r in rows]
+        out["solvers"][name] = {
+            "median_iters": float(np.median(its)), "median_ms": float(np.median(ms)),
+            "max_true_residual": max(r[name]["true_residual"] for r in rows),
+            "fallbacks": sum(r[name]["fallback"] is not None for r in rows)}
+    base_ms = out["solvers"]["cgls"]["median_ms"]
+    for name in preconds:
+        out["solvers"][name]["time_ratio_vs_cgls"] = out["solvers"][name]["median_ms"] / base_ms
+    out["rows"] = rows
+    return out
+
+
+def save_preconditioner(path, pre: CnnPreconditioner, meta: dict) -> None:
+    """Network weights (FP32 master copies) and the base's settings as .npz."""
+    arrays = {k: v.numpy() for k, v in pre.state_dict().items()}
+    arrays["meta_keys"] = np.array(list(meta.keys()))
+    arrays["meta_vals"] = np.array([float(v) for v in meta.values()])
+    np.savez(path, **arrays)
+
+
+def load_preconditioner(path, op, side: int) -> CnnPreconditioner:
+    """Rebuild M = D [z + s net(z / s)] for `op` from `save_preconditioner` output."""
+    with np.load(Path(path), allow_pickle=False) as f:
+        meta = dict(zip([str(k) for k in f["meta_keys"]], f["meta_vals"].tolist()))
+        sd = {k: torch.from_numpy(f[k]) for k in f.files if k[0] in "wb" and k[1:].isdigit()}
+    pad = int(meta.get("pad", side))
+    base = CirculantPreconditioner(op, side, meta["tau"], pad=pad)
+    pre = CnnPreconditioner(op.device, base=base)
+    pre.load_state_dict(sd)
+    return pre

@@ -9,8 +9,9 @@ each iteration synchronises once to read the residual norm.
   solve is scored by `true_residual`, recomputed from scratch.
 * `fpcg`: flexible PCG with a preconditioner callable z = M(r) that may be
   nonlinear (a trained network): Polak-Ribiere beta.  Non-finite values or
-  non-positive curvature p^H K p <= 0 switch the solve to CGLS from the
-  current iterate (reported in the result).
+  non-positive curvature p^H K p <= 0, or a residual grown to 10x the
+  smallest seen so far, switch the solve to CGLS from the best iterate
+  (reported in the result).
 Stopping test: ||A^H b - (A^H A + lam I) x|| / ||A^H b|| <= tol.
 """
 from __future__ import annotations
@@ -116,14 +117,17 @@ def fpcg(op, b: torch.Tensor, precond: Callable[[torch.Tensor], torch.Tensor], l
     it = 0
     fallback = None
     converged = False
+    best_x, best_res = x, 1.0
     while it < max_iter:
         if not bool(torch.isfinite(torch.view_as_real(z) if z.is_complex() else z).all()):
             fallback = "preconditioner returned non-finite values"
+            x = best_x
             break
         Kp = normal_apply(op, p, lam)
         pKp = _dot(p, Kp)
         if not (float(pKp.real) > 0.0) or not math.isfinite(float(pKp.real)):
             fallback = "non-positive curvature p^H K p <= 0"
+            x = best_x
             break
         alpha = (rz / pKp).to(x.dtype)
         x = x + alpha * p
@@ -133,6 +137,12 @@ def fpcg(op, b: torch.Tensor, precond: Callable[[torch.Tensor], torch.Tensor], l
         if res[-1] <= tol:
             converged = True
             r = r_new
+            break
+        if res[-1] < best_res:
+            best_x, best_res = x, res[-1]
+        elif not math.isfinite(res[-1]) or res[-1] > 10.0 * best_res:
+            fallback = "residual grew to 10x its minimum"
+            x = best_x
             break
         z_new = precond(r_new)
         rz_new = _dot(r_new, z_new)

@@ -107,3 +107,45 @@ def test_layer_backward_matches_torch(dev):
     print(f"  dx {ex:.2e}  dW {ew:.2e}  db {eb:.2e}")
     # the ReLU mask comes from the bf16 output and dx is rounded to bf16
     assert ex <= 1e-2 and ew <= 1e-2 and eb <= 1e-2
+
+
+def test_learned_preconditioner_gain(dev):
+    """What the network gains, on the 128^2 workload (256 pulses x 512 samples,
+    2 deg, U = 8): CGLS, PCG with the demodulated circulant base alone, and PCG
+    with the base plus the CNN after a short unrolled training, on 8 held-out
+    20 dB scenes to relative residual 1e-3.  Every solve must reach the
+    tolerance (true residual, recomputed); the trained network must not lose
+    to the base on the held-out unrolled loss by more than training noise and
+    the learned PCG must not need more iterations than CGLS.  The gains are
+    printed."""
+    from sarsolver_amd import learned
+    from sarsolver_amd.train_preconditioner import workload_operator
+    print("learned preconditioner gain on c1_small", flush=True)
+    w, op = workload_operator("c1_small", dev)
+    n, lam = w.n, 1e-3
+    gens = learned.make_gens(11, dev)
+    with torch.no_grad():
+        probs = [learned.noisy_problem(op, x, 20.0, gens.dev_gen)
+                 for x in learned.random_scenes(n, 4, gens, dev)]
+    tau, scores = learned.choose_tau(op, n, (0.03, 0.08, 0.2), probs, lam=lam, pad=2 * n)
+    base = learned.CirculantPreconditioner(op, n, tau, pad=2 * n)
+    pre = learned.CnnPreconditioner(dev, base=base)
+    cval = learned.problem_batch(op, n, 4, learned.make_gens(12, dev))
+    M = lambda r: pre(r, (n, n))
+    with torch.no_grad():
+        l0 = float(learned.unrolled_pcg_loss(op, M, cval, lam, 8))
+    learned.train_unrolled(op, pre, n, lam=lam, iters=8, steps=40, batch=2, lr=1e-4)
+    with torch.no_grad():
+        l1 = float(learned.unrolled_pcg_loss(op, M, cval, lam, 8))
+    res = learned.evaluate(op, {"circulant": base, "learned": M}, n, scenes=8, lam=lam)
+    s = res["solvers"]
+    print(f"  tau {tau} (scores {scores}); unrolled loss base {l0:.4f} -> learned {l1:.4f}")
+    for name, v in s.items():
+        print(f"  {name:9s} median iters {v['median_iters']:5.1f}  median ms {v['median_ms']:7.2f}"
+              f"  worst true residual {v['max_true_residual']:.2e}  fallbacks {v['fallbacks']}")
+    print(f"  network gain over base: {s['circulant']['median_iters'] - s['learned']['median_iters']:+.1f}"
+          f" iterations; learned/CGLS time {s['learned']['time_ratio_vs_cgls']:.2f}")
+    for v in s.values():
+        assert v["max_true_residual"] <= 1.05e-3
+    assert l1 <= l0 + 0.05
+    assert s["learned"]["median_iters"] <= s["cgls"]["median_iters"]

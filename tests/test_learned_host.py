@@ -75,3 +75,89 @@ def test_conv_entry_points_reject_bad_arguments_without_a_gpu():
     L.sarhip_conv3x3_wgrad_workspace.restype = ctypes.c_size_t
     assert L.sarhip_conv3x3_wgrad_workspace(2, 33, 17, 32) == 2 * 3 * 2 * (9 * 32 * 32 + 32) * 4
     assert L.sarhip_conv3x3_wgrad_workspace(1, 16, 16, 2) == (9 * 32 * 16 + 16) * 4
+
+
+class _CirculantOp:
+    """A = S F D^H on n x n images (F the unitary 2-D DFT, S = diag(sqrt(sigma)),
+    D = diag(exp(i phi))): A^H A = D F^H diag(sigma) F D^H, a demodulated
+    circulant.  CPU, complex128."""
+
+    def __init__(self, n, sigma, phi):
+        self.n, self.device = n, torch.device("cpu")
+        self.s = torch.sqrt(torch.as_tensor(sigma, dtype=torch.float64))
+        self.d = torch.polar(torch.ones(n * n, dtype=torch.float64),
+                             torch.as_tensor(phi, dtype=torch.float64))
+
+    def forward(self, x):
+        z = (x.to(torch.complex128) * self.d.conj()).reshape(self.n, self.n)
+        return (self.s * torch.fft.fft2(z, norm="ortho")).reshape(-1)
+
+    def adjoint(self, y):
+        z = torch.fft.ifft2(self.s * y.reshape(self.n, self.n), norm="ortho").reshape(-1)
+        return z * self.d
+
+
+def _band_sigma(n):
+    f = torch.fft.fftfreq(n, dtype=torch.float64)
+    return 0.05 + torch.exp(-((f[:, None] / 0.15) ** 2 + (f[None, :] / 0.1) ** 2))
+
+
+def test_circulant_preconditioner_inverts_a_demodulated_circulant():
+    """For an operator whose normal matrix is exactly D C D^H, the base with
+    tau = lam and no padding is (A^H A + lam I)^-1: the PSF measured through the
+    operator, the re-centring roll, the demodulation sign and the filter all
+    have to be right for K P r = r."""
+    from sarsolver_amd.learned import CirculantPreconditioner
+    n, lam = 32, 1e-3
+    yy, xx = np.mgrid[0:n, 0:n]
+    phi = (0.9 * yy + 0.03 * xx ** 2 + 0.4 * xx).reshape(-1)   # carrier with curvature
+    op = _CirculantOp(n, _band_sigma(n), phi)
+    P = CirculantPreconditioner(op, n, lam, pad=n, phase=phi)
+    assert torch.allclose(P.demod.to(torch.complex128), op.d, atol=1e-6)
+    g = torch.Generator().manual_seed(3)
+    r = torch.complex(torch.randn(n * n, generator=g, dtype=torch.float64),
+                      torch.randn(n * n, generator=g, dtype=torch.float64))
+    z = P(r.to(torch.complex64)).to(torch.complex128)
+    Kz = op.adjoint(op.forward(z)) + lam * z
+    assert float(torch.linalg.vector_norm(Kz - r) / torch.linalg.vector_norm(r)) < 1e-5
+
+
+def test_circulant_preconditioner_picks_the_demodulation_sign():
+    """With the conjugate carrier the base must select the opposite sign."""
+    from sarsolver_amd.learned import CirculantPreconditioner
+    n = 32
+    yy, xx = np.mgrid[0:n, 0:n]
+    phi = (1.3 * yy + 0.02 * yy ** 2 + 0.7 * xx).reshape(-1)
+    op = _CirculantOp(n, _band_sigma(n), -phi)
+    P = CirculantPreconditioner(op, n, 1e-3, pad=n, phase=phi)
+    assert torch.allclose(P.demod.to(torch.complex128), op.d, atol=1e-6)
+
+
+def test_unrolled_loss_with_identity_is_plain_cg():
+    """With M = I the unrolled recursion is CG on the normal equations, whose
+    residuals CGLS reports; the loss is the mean log10 of the first `iters`."""
+    from sarsolver_amd.learned import unrolled_pcg_loss
+    from sarsolver_amd.solvers import DenseOperator, cgls
+    g = torch.Generator().manual_seed(5)
+    A = torch.complex(torch.randn(60, 40, generator=g, dtype=torch.float64),
+                      torch.randn(60, 40, generator=g, dtype=torch.float64))
+    b = torch.complex(torch.randn(60, generator=g, dtype=torch.float64),
+                      torch.randn(60, generator=g, dtype=torch.float64))
+    op = DenseOperator(A)
+    lam, T = 0.01, 6
+    ref = cgls(op, b, lam=lam, tol=0.0, max_iter=T)
+    c = op.adjoint(b)[None, :]
+    loss = float(unrolled_pcg_loss(op, lambda r: r, c, lam, T))
+    assert abs(loss - float(np.mean(np.log10(ref.residuals[1:T + 1])))) < 1e-8
+
+
+def test_weights_round_trip(tmp_path):
+    from sarsolver_amd.learned import CnnPreconditioner, save_preconditioner
+    pre = CnnPreconditioner(torch.device("cpu"), seed=4)
+    with torch.no_grad():
+        pre.weights[-1].normal_()
+    save_preconditioner(tmp_path / "m.npz", pre, {"tau": 0.05, "pad": 64})
+    with np.load(tmp_path / "m.npz", allow_pickle=False) as f:
+        assert dict(zip(f["meta_keys"].tolist(), f["meta_vals"].tolist())) == {"tau": 0.05, "pad": 64.0}
+        for i, w in enumerate(pre.weights):
+            assert np.array_equal(f[f"w{i}"], w.detach().numpy())

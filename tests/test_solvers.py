@@ -78,3 +78,20 @@ def test_fpcg_falls_back_to_cgls(bad, reason):
     assert res.fallback is not None and reason in res.fallback
     assert res.converged
     assert true_residual(DenseOperator(A), res.x, b, 1e-3) <= 1e-7
+
+
+def test_fpcg_hands_over_to_cgls_when_the_residual_grows():
+    """A preconditioner that rotates residuals by i points every step uphill; the
+    solve must detect the growth, restart from the best iterate with CGLS and
+    still converge."""
+    torch = pytest.importorskip("torch")
+    from sarsolver_amd.solvers import DenseOperator, fpcg, true_residual
+    g = torch.Generator().manual_seed(9)
+    A = torch.complex(torch.randn(50, 30, generator=g, dtype=torch.float64),
+                      torch.randn(50, 30, generator=g, dtype=torch.float64))
+    b = torch.complex(torch.randn(50, generator=g, dtype=torch.float64),
+                      torch.randn(50, generator=g, dtype=torch.float64))
+    op = DenseOperator(A)
+    res = fpcg(op, b, lambda r: 1j * r, lam=0.1, tol=1e-8, max_iter=200)
+    assert res.fallback is not None
+    assert res.converged and true_residual(op, res.x, b, 0.1) <= 1e-7
