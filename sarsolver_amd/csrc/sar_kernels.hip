@@ -23,6 +23,11 @@ namespace sarhip {
 // Spectral stages (HBM-bound elementwise kernels)
 // ---------------------------------------------------------------------------
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+// Word 3 of a raw buffer resource on gfx9-family GPUs (32-bit data format,
+// no swizzle, no stride).
+constexpr int kBufferWord3 = 0x00020000;
+
 // Two consecutive profile samples (8-byte aligned) read as one 16-byte load.
 struct alignas(8) Float2x2 {
     float a, b, c, d;
@@ -40,7 +45,7 @@ __device__ __forceinline__ float2 cmul(const float2 a, const float2 b) {
 __global__ __launch_bounds__(256) void window_pad_kernel(const float2 *__restrict__ Y,
                                                          const float2 *__restrict__ Wconj,
                                                          const float2 *__restrict__ wfac, int N,
-                                                         int nw, int centred,
+                                                         int nw, int ld, int centred,
                                                          float2 *__restrict__ K) {
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int p = blockIdx.y;
@@ -55,7 +60,18 @@ __global__ __launch_bounds__(256) void window_pad_kernel(const float2 *__restric
         const float2 y = Y[(size_t)p * N + src];
         v = cmul(cmul(Wconj[k], y), wfac[p]);
     }
-    K[(size_t)p * nw + j] = v;
+    K[(size_t)p * ld + j] = v;
+}
+
+// Row tails of the upsampled profiles: prof[p][nw + j] = prof[p][j mod nw] for
+// j < ld - nw, so a gather can read bins n and n + 1 for any n < ld - 1
+// without wrapping.
+__global__ __launch_bounds__(256) void wrap_pad_kernel(float2 *__restrict__ prof, int nw, int ld) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int pad = ld - nw;
+    if (j >= pad) return;
+    float2 *row = prof + (size_t)blockIdx.y * ld;
+    row[nw + j] = row[j % nw];
 }
 
 // Y[p, k] = W[k] * wfac[p] * G[p, src(k)] (CPP:295-307); `centred` output
@@ -196,7 +212,8 @@ __device__ __forceinline__ void load_pulse(const KernelParams &k, int tile, int 
 }
 
 // u = dR * by_dx (bins), th = dR * k_c / 2 pi (revolutions).
-template <int PREC, bool MONO>
+// PLANAR: every pixel offset has dz == 0 (the z terms are dropped).
+template <int PREC, bool MONO, bool PLANAR = false>
 __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseData<MONO> &d,
                                            const Pix &q, double &u, double &th) {
     if constexpr (PREC == kPrecFp64) {
@@ -205,15 +222,16 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
         th = dr * k.kc_rev;
     } else if constexpr (PREC == kPrecSeries) {
         if constexpr (MONO) {
-            const double t = fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
+            const double t = PLANAR ? fma(d.s.bx, q.dx, fma(d.s.by, q.dy, d.s.c * q.d2))
+                                    : fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
             const double g = sqrt1pm1_series(t);
             u = fma(d.s.ku, g, d.s.cu);
             th = fma(d.s.kt, g, d.s.ct);
         } else {
-            const double tt =
-                fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, fma(d.s.tbz, q.dz, d.s.tc * q.d2)));
-            const double tr =
-                fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, fma(d.s.rbz, q.dz, d.s.rc * q.d2)));
+            const double zt = PLANAR ? d.s.tc * q.d2 : fma(d.s.tbz, q.dz, d.s.tc * q.d2);
+            const double zr = PLANAR ? d.s.rc * q.d2 : fma(d.s.rbz, q.dz, d.s.rc * q.d2);
+            const double tt = fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, zt));
+            const double tr = fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, zr));
             const double gt = sqrt1pm1_series(tt), gr = sqrt1pm1_series(tr);
             u = fma(d.s.kut, gt, fma(d.s.kur, gr, d.s.cu));
             th = fma(d.s.ktt, gt, fma(d.s.ktr, gr, d.s.ct));
@@ -222,16 +240,16 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
         // FP64 tile-centre range (cu, ct) + FP32 differential range.
         const float dx = (float)q.dx, dy = (float)q.dy, dz = (float)q.dz, d2 = (float)q.d2;
         if constexpr (MONO) {
-            const float t = fmaf((float)d.s.bx, dx,
-                                 fmaf((float)d.s.by, dy, fmaf((float)d.s.bz, dz, (float)d.s.c * d2)));
+            const float tz = PLANAR ? (float)d.s.c * d2 : fmaf((float)d.s.bz, dz, (float)d.s.c * d2);
+            const float t = fmaf((float)d.s.bx, dx, fmaf((float)d.s.by, dy, tz));
             const float g = sqrt1pm1_series_f(t);
             u = d.s.cu + (double)((float)d.s.ku * g);
             th = d.s.ct + (double)((float)d.s.kt * g);
         } else {
-            const float tt = fmaf((float)d.s.tbx, dx,
-                                  fmaf((float)d.s.tby, dy, fmaf((float)d.s.tbz, dz, (float)d.s.tc * d2)));
-            const float tr = fmaf((float)d.s.rbx, dx,
-                                  fmaf((float)d.s.rby, dy, fmaf((float)d.s.rbz, dz, (float)d.s.rc * d2)));
+            const float zt = PLANAR ? (float)d.s.tc * d2 : fmaf((float)d.s.tbz, dz, (float)d.s.tc * d2);
+            const float zr = PLANAR ? (float)d.s.rc * d2 : fmaf((float)d.s.rbz, dz, (float)d.s.rc * d2);
+            const float tt = fmaf((float)d.s.tbx, dx, fmaf((float)d.s.tby, dy, zt));
+            const float tr = fmaf((float)d.s.rbx, dx, fmaf((float)d.s.rby, dy, zr));
             const float gt = sqrt1pm1_series_f(tt), gr = sqrt1pm1_series_f(tr);
             u = d.s.cu + (double)fmaf((float)d.s.kut, gt, (float)d.s.kur * gr);
             th = d.s.ct + (double)fmaf((float)d.s.ktt, gt, (float)d.s.ktr * gr);
@@ -294,46 +312,55 @@ __device__ __forceinline__ int bin_index(const KernelParams &k, double n, int sp
 // or per-(tile, pulse) constants come through scalar loads.  Profile samples
 // are gathered from the upsampled range profiles (L1/L2-resident windows).
 // Phase in FP64 revolutions, integer part removed, FP32 sin/cos.  Products and
-// sums in FP32; per-pixel sums are folded into FP64 every 64 pulses.
+// sums in FP32; every 64 pulses the per-pixel FP32 sums are folded into FP64
+// sums kept in LDS (out of registers, for occupancy).
 // ---------------------------------------------------------------------------
-template <int INTERP, bool MONO, int PREC>
+template <int INTERP, bool MONO, int PREC, bool PLANAR>
 __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
+    __shared__ double acc[2 * kPixPerThread][kThreads];
     const int tile = blockIdx.x;
+    const int tid = (int)threadIdx.x;
     const double4 X0 = k.tile_c[tile];
     Pix q[kPixPerThread];
     float ar[kPixPerThread], ai[kPixPerThread];
-    double sr[kPixPerThread], si[kPixPerThread];
 #pragma unroll
     for (int m = 0; m < kPixPerThread; ++m) {
-        q[m] = load_pix(k, tile * kTilePixels + m * kThreads + (int)threadIdx.x, X0);
+        q[m] = load_pix(k, tile * kTilePixels + m * kThreads + tid, X0);
         ar[m] = ai[m] = 0.f;
-        sr[m] = si[m] = 0.0;
+        acc[2 * m][tid] = acc[2 * m + 1][tid] = 0.0;
     }
     for (int j = 0; j < k.Pv; ++j) {
         const int p = k.rows[j];
         PulseData<MONO> d;
         load_pulse<PREC, MONO>(k, tile, j, p, d);
-        const float2 *row = k.prof + (size_t)p * k.nw;
+        const float2 *row = k.prof + (size_t)p * k.ld;
+        // 16-byte loads of bins (n, n + 1) through a buffer resource over the
+        // padded row: offsets beyond it read zeros instead of faulting.
+        const __amdgpu_buffer_rsrc_t rsrc =
+            __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0, k.ld * 8, kBufferWord3);
+        constexpr bool kShift = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
+        if constexpr (kShift) {
+            // Move u by a multiple of nw so that every pixel of the tile lands
+            // in [0, nw + 2 shift_lo): u(tile centre) - shift_lo is brought
+            // into [0, nw).  Bins are unchanged modulo nw and the padded row
+            // tail makes bins n + 1 <= ld - 1 valid without a wrap.
+            d.s.cu = fma(-floor((d.s.cu - k.shift_lo) * k.inv_nw), k.nwd, d.s.cu);
+        }
 #pragma unroll
         for (int m = 0; m < kPixPerThread; ++m) {
             double u, th;
-            eval_range<PREC, MONO>(k, d, q[m], u, th);
+            eval_range<PREC, MONO, PLANAR>(k, d, q[m], u, th);
             float hr, hi;
             if constexpr (INTERP == kInterpLinear) {
                 const double fl = floor(u);
                 const float fr = (float)(u - fl);
-                const int n0 = bin_index(k, fl, 2);
-                float2 h0, h1;
-                if (n0 + 1 < k.nw) {
-                    const Float2x2 h = *(const Float2x2 *)(row + n0);
-                    h0 = make_float2(h.a, h.b);
-                    h1 = make_float2(h.c, h.d);
-                } else {
-                    h0 = row[n0];
-                    h1 = row[0];
-                }
-                hr = fmaf(fr, h1.x - h0.x, h0.x);
-                hi = fmaf(fr, h1.y - h0.y, h0.y);
+                int n0;
+                if constexpr (kShift) n0 = (int)fl;
+                else n0 = bin_index(k, fl, 1);
+                const u32x4 h = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
+                const float h0x = __uint_as_float(h.x), h0y = __uint_as_float(h.y);
+                hr = fmaf(fr, __uint_as_float(h.z) - h0x, h0x);
+                hi = fmaf(fr, __uint_as_float(h.w) - h0y, h0y);
             } else {
                 const double n = nearest_bin<INTERP, PREC, MONO>(k, q[m], p, u, th);
                 const float2 h = row[bin_index(k, n, 1)];
@@ -343,24 +370,24 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
             float s, c;
             phasor_rev(th, &s, &c);
             // h * e^{-i theta}
-            ar[m] += fmaf(hr, c, hi * s);
-            ai[m] += fmaf(hi, c, -(hr * s));
+            ar[m] = fmaf(hr, c, fmaf(hi, s, ar[m]));
+            ai[m] = fmaf(hi, c, fmaf(-hr, s, ai[m]));
         }
         if ((j & 63) == 63) {
 #pragma unroll
             for (int m = 0; m < kPixPerThread; ++m) {
-                sr[m] += (double)ar[m];
-                si[m] += (double)ai[m];
+                acc[2 * m][tid] += (double)ar[m];
+                acc[2 * m + 1][tid] += (double)ai[m];
                 ar[m] = ai[m] = 0.f;
             }
         }
     }
 #pragma unroll
     for (int m = 0; m < kPixPerThread; ++m) {
-        const int i = tile * kTilePixels + m * kThreads + (int)threadIdx.x;
+        const int i = tile * kTilePixels + m * kThreads + tid;
         if (i < k.M) {
-            const double vr = q[m].ok ? sr[m] + (double)ar[m] : 0.0;
-            const double vi = q[m].ok ? si[m] + (double)ai[m] : 0.0;
+            const double vr = q[m].ok ? acc[2 * m][tid] + (double)ar[m] : 0.0;
+            const double vi = q[m].ok ? acc[2 * m + 1][tid] + (double)ai[m] : 0.0;
             k.out[k.perm[i]] = make_float2((float)vr, (float)vi);
         }
     }
@@ -654,9 +681,16 @@ __global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams 
 // ---------------------------------------------------------------------------
 
 hipError_t launch_window_pad(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N,
-                             int nw, int P, int centred, float2 *K, hipStream_t s) {
+                             int nw, int ld, int P, int centred, float2 *K, hipStream_t s) {
     dim3 grid((nw + 255) / 256, P);
-    window_pad_kernel<<<grid, 256, 0, s>>>(Y, Wconj, wfac, N, nw, centred, K);
+    window_pad_kernel<<<grid, 256, 0, s>>>(Y, Wconj, wfac, N, nw, ld, centred, K);
+    return hipGetLastError();
+}
+
+hipError_t launch_wrap_pad(float2 *prof, int nw, int ld, int P, hipStream_t s) {
+    if (ld <= nw || P <= 0) return hipSuccess;
+    dim3 grid((ld - nw + 255) / 256, P);
+    wrap_pad_kernel<<<grid, 256, 0, s>>>(prof, nw, ld);
     return hipGetLastError();
 }
 
@@ -699,11 +733,19 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
 template <int I, bool MO>
 static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const dim3 grid(k.num_tiles);
+    // planar specialisations for the linear series / hilo paths only
+    const bool pl = I == kInterpLinear && k.planar;
     switch (k.precision) {
-        case kPrecSeries: bp_gather_kernel<I, MO, kPrecSeries><<<grid, kThreads, 0, s>>>(k); break;
-        case kPrecHilo: bp_gather_kernel<I, MO, kPrecHilo><<<grid, kThreads, 0, s>>>(k); break;
-        case kPrecFp32: bp_gather_kernel<I, MO, kPrecFp32><<<grid, kThreads, 0, s>>>(k); break;
-        default: bp_gather_kernel<I, MO, kPrecFp64><<<grid, kThreads, 0, s>>>(k); break;
+        case kPrecSeries:
+            if (pl) bp_gather_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, 0, s>>>(k);
+            else bp_gather_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, 0, s>>>(k);
+            break;
+        case kPrecHilo:
+            if (pl) bp_gather_kernel<I, MO, kPrecHilo, true><<<grid, kThreads, 0, s>>>(k);
+            else bp_gather_kernel<I, MO, kPrecHilo, false><<<grid, kThreads, 0, s>>>(k);
+            break;
+        case kPrecFp32: bp_gather_kernel<I, MO, kPrecFp32, false><<<grid, kThreads, 0, s>>>(k); break;
+        default: bp_gather_kernel<I, MO, kPrecFp64, false><<<grid, kThreads, 0, s>>>(k); break;
     }
 }
 

@@ -25,6 +25,10 @@ struct KernelParams {
     float2 *prof;
     int nw;
     double nwd, inv_nw, by_dx, kc_rev;
+    // adjoint profile rows: stride ld = nw + pad, bins [nw, ld) repeat bins
+    // [0, ld - nw) (mod nw); shift_lo bounds |u - u(tile centre)| in bins
+    int ld;
+    double shift_lo;
     // adjoint output (caller order)
     float2 *out;
     // forward
@@ -42,10 +46,12 @@ struct KernelParams {
     // variant
     int interp, precision;
     bool mono;
+    bool planar;  // every pixel's offset from its tile centre has dz == 0
 };
 
 hipError_t launch_window_pad(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N,
-                             int nw, int P, int centred, float2 *K, hipStream_t s);
+                             int nw, int ld, int P, int centred, float2 *K, hipStream_t s);
+hipError_t launch_wrap_pad(float2 *prof, int nw, int ld, int P, hipStream_t s);
 hipError_t launch_truncate(const float2 *G, const float2 *W, const float2 *wfac, int N, int nw,
                            int P, int centred, float2 *Y, hipStream_t s);
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s);

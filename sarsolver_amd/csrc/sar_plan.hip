@@ -102,10 +102,13 @@ struct sarhip_plan_s {
     double by_dx = 0, kc = 0, kc_rev = 0, inv_nw = 0;
     int interp = 0, precision = 0, requested_precision = 0;
     bool mono = false;
+    bool planar = false;
     int Pv = 0;                  // valid pulses
     int num_tiles = 0, win = 0, reduce_chunk = 0;
     int num_regions = 0, region_tiles = 1, fwd_pulses = 32, fwd_groups = 1;
     double series_err_bins = 0;  // bound on the series truncation term (bins)
+    size_t ld = 0;               // adjoint profile row stride (nw + padded tail)
+    double shift_lo = 0;         // bins: bound on |u - u(tile centre)| plus one
 
     PulseConst *d_pc = nullptr;
     int *d_rows = nullptr;
@@ -179,6 +182,8 @@ struct sarhip_plan_s {
         k.series = d_series;
         k.prof = d_prof;
         k.nw = (int)nw;
+        k.ld = (int)ld;
+        k.shift_lo = shift_lo;
         k.nwd = (double)nw;
         k.inv_nw = inv_nw;
         k.by_dx = by_dx;
@@ -198,6 +203,7 @@ struct sarhip_plan_s {
         k.interp = interp;
         k.precision = precision;
         k.mono = mono;
+        k.planar = planar;
         return k;
     }
 };
@@ -488,6 +494,17 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     double rad_t = 0.0;
     make_tiles(sx, sy, sz, M, kTilePixels, true, tiles, rad_t);
     pl->num_tiles = (int)tiles.size();
+    // Within a tile of radius r each leg's range moves by at most r (triangle
+    // inequality), so |u - u(tile centre)| <= 2 r by_dx; series truncation and
+    // rounding add far less than the one-bin slack.  Adjoint rows get a tail
+    // of ceil(2 shift_lo) + 2 bins (see bp_gather_kernel).
+    pl->shift_lo = 2.0 * rad_t * pl->by_dx * (1.0 + 1e-6) + 2.0;
+    {
+        const double tail = std::ceil(2.0 * pl->shift_lo) + 2.0;
+        if (!(tail < 1e9) || (double)nw + tail > 2147483647.0 / 8.0)
+            return fail(SARHIP_EINVAL, "scene tiles too large for the padded profile rows");
+        pl->ld = nw + (size_t)tail;
+    }
     std::vector<PixRec> recs(M);
     for (size_t i = 0; i < M; ++i) {
         const double4 &c = tiles[i / kTilePixels];
@@ -502,6 +519,8 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         r.d2 = std::fma(r.dx, r.dx, std::fma(r.dy, r.dy, r.dz * r.dz));
         r.ok = fin ? 1.0 : 0.0;
     }
+    pl->planar = true;
+    for (size_t i = 0; i < M && pl->planar; ++i) pl->planar = recs[i].dz == 0.0;
 
     // Series modes need the truncation term far below the tie margin; if the
     // geometry does not allow it the series mode falls back to exact FP64.
@@ -606,7 +625,7 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     if ((rc = upload((void **)&pl->d_Wconj, Wc.data(), sizeof(float2) * N))) return rc;
     if ((rc = upload((void **)&pl->d_wfac_fwd, wf.data(), sizeof(float2) * P))) return rc;
     if ((rc = upload((void **)&pl->d_wfac_adj, wa.data(), sizeof(float2) * P))) return rc;
-    HIP_TRY(hipMalloc((void **)&pl->d_prof, sizeof(float2) * P * nw));
+    HIP_TRY(hipMalloc((void **)&pl->d_prof, sizeof(float2) * P * pl->ld));
     HIP_TRY(hipMalloc((void **)&pl->d_amps_perm, sizeof(float2) * M));
     const size_t nrows = std::max<size_t>(1, (size_t)pl->Pv);
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
@@ -626,9 +645,21 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     FFT_TRY(rocfft_plan_create(&pl->fft_fwd, rocfft_placement_inplace,
                                rocfft_transform_type_complex_forward, rocfft_precision_single, 1,
                                &len, P, nullptr));
-    FFT_TRY(rocfft_plan_create(&pl->fft_bwd, rocfft_placement_inplace,
-                               rocfft_transform_type_complex_inverse, rocfft_precision_single, 1,
-                               &len, P, nullptr));
+    {
+        // adjoint: rows of stride ld (padded tails), in place
+        rocfft_plan_description desc = nullptr;
+        FFT_TRY(rocfft_plan_description_create(&desc));
+        const size_t stride = 1, dist = pl->ld;
+        rocfft_status st = rocfft_plan_description_set_data_layout(
+            desc, rocfft_array_type_complex_interleaved, rocfft_array_type_complex_interleaved,
+            nullptr, nullptr, 1, &stride, dist, 1, &stride, dist);
+        if (st == rocfft_status_success)
+            st = rocfft_plan_create(&pl->fft_bwd, rocfft_placement_inplace,
+                                    rocfft_transform_type_complex_inverse, rocfft_precision_single,
+                                    1, &len, P, desc);
+        rocfft_plan_description_destroy(desc);
+        FFT_TRY(st);
+    }
     size_t w1 = 0, w2 = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl->fft_fwd, &w1));
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl->fft_bwd, &w2));
@@ -744,10 +775,11 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     if ((rc = begin_timing(pl, 1))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
     HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
-                              (int)pl->N, (int)pl->nw, (int)pl->P,
+                              (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,
                               order == SARHIP_ORDER_CENTRED, pl->d_prof, s));
     if ((rc = record(pl, 1, s))) return rc;
     if ((rc = run_fft(pl, pl->fft_bwd, pl->d_prof, s))) return rc;
+    HIP_TRY(launch_wrap_pad(pl->d_prof, (int)pl->nw, (int)pl->ld, (int)pl->P, s));
     if ((rc = record(pl, 2, s))) return rc;
     KernelParams k = pl->params();
     k.out = (float2 *)amps;
