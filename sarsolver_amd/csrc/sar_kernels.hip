@@ -421,11 +421,48 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
 // expected) falls back to a global atomic into the profile.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int fix_rn(const float v) { return (int)rintf(v); }
+// (re, im) -> im * 2^32 + re as a 64-bit two's-complement word
+__device__ __forceinline__ unsigned long long pack_fix(const int re, const int im) {
+    return (unsigned long long)(((long long)im << 32) + (long long)re);
+}
+// inverse of a sum of pack_fix words whose component sums lie in (-2^31, 2^31)
+__device__ __forceinline__ int2 unpack_fix(const unsigned long long v) {
+    const int re = (int)(unsigned)(v & 0xffffffffull);
+    const int im = (int)(((long long)v - (long long)re) >> 32);
+    return make_int2(re, im);
+}
 
-template <int INTERP, bool MONO, int PREC>
+// Per-region L1 norm sum(|Re a| + |Im a|) of the permuted amplitudes (FP32,
+// fixed reduction order), for the forward's fixed-point scale.
+__global__ __launch_bounds__(kThreads) void region_l1_kernel(const float2 *__restrict__ amps,
+                                                             int M, int region_pixels,
+                                                             float *__restrict__ out) {
+    __shared__ float red[kThreads];
+    const int base = blockIdx.x * region_pixels;
+    const int n = min(region_pixels, M - base);
+    float l1 = 0.f;
+    for (int i = threadIdx.x; i < n; i += kThreads) {
+        const float2 a = amps[base + i];
+        l1 += fabsf(a.x) + fabsf(a.y);
+    }
+    red[threadIdx.x] = l1;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+template <int INTERP, bool MONO, int PREC, bool PLANAR>
 __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams k) {
-    extern __shared__ int2 wsum[];  // [G][win] fixed point (re, im)
-    __shared__ float l1_red[kThreads];
+    // Fixed-point windows [G][wst], one 64-bit word per bin holding
+    // im * 2^32 + re: a single 64-bit LDS add accumulates both components
+    // (the low word's carries land in the high word, and both parts are
+    // recovered exactly because each stays below 2^31 in magnitude).  The
+    // per-pulse stride wst is odd, spreading a wave's lanes (distinct pulses,
+    // nearby bins) over the LDS banks.
+    extern __shared__ unsigned long long wsum[];
     __shared__ int wmax[kThreads];  // bit patterns of non-negative floats, [G]
     const int G = k.fwd_pulses, R = k.region_tiles;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
@@ -446,22 +483,9 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     const int win = k.win;
     const int t0 = region * R;
     const int t1 = min(k.num_tiles, t0 + R);
-    const int base = t0 * kTilePixels;
-    const int npix = min(R * kTilePixels, k.M - base);
 
-    // region L1 norm -> fixed-point scale
-    float l1 = 0.f;
-    for (int i = tid; i < npix; i += kThreads) {
-        const float2 a = k.amps_perm[base + i];
-        l1 += fabsf(a.x) + fabsf(a.y);
-    }
-    l1_red[tid] = l1;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) l1_red[tid] += l1_red[tid + s];
-        __syncthreads();
-    }
-    l1 = l1_red[0];
+    // region L1 norm (region_l1_kernel) -> fixed-point scale
+    const float l1 = k.region_l1[region];
     const bool fixed_ok = isfinite(l1) && win > 0;
     const float scale = (fixed_ok && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
@@ -488,9 +512,10 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     }
     const double wb =
         floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - sleg * (1.0 + 1e-9) - 1e-9) * k.by_dx) - 2.0;
-    for (int e = tid; e < G * win; e += kThreads) wsum[e] = make_int2(0, 0);
+    const int wst = k.win_stride;
+    for (int e = tid; e < G * wst; e += kThreads) wsum[e] = 0ull;
     __syncthreads();
-    int2 *ws = wsum + q * win;
+    unsigned long long *ws = wsum + q * wst;
     float2 *row = k.prof + (size_t)p * k.nw;
     if (!(fixed_ok && scale == 0.f)) {  // skip all-zero regions
         for (int tile = t0; tile < t1; ++tile) {
@@ -512,7 +537,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
                 px.d2 = r.d2;
                 px.ok = true;
                 double u, th;
-                eval_range<PREC, MONO>(k, d, px, u, th);
+                eval_range<PREC, MONO, PLANAR>(k, d, px, u, th);
                 float sn, cs;
                 double fl;
                 if constexpr (INTERP == kInterpLinear) {
@@ -532,10 +557,8 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
                             const float sr = cr * scale, si = ci * scale;
                             const int tr = fix_rn(sr), ti = fix_rn(si);
                             const int r1 = fix_rn(fr * sr), i1 = fix_rn(fr * si);
-                            atomicAdd(&ws[l].x, tr - r1);
-                            atomicAdd(&ws[l].y, ti - i1);
-                            atomicAdd(&ws[l + 1].x, r1);
-                            atomicAdd(&ws[l + 1].y, i1);
+                            atomicAdd(&ws[l], pack_fix(tr - r1, ti - i1));
+                            atomicAdd(&ws[l + 1], pack_fix(r1, i1));
                         }
                     } else if (qok) {
                         const float gr = 1.0f - fr;
@@ -549,8 +572,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
                 } else {
                     if (fixed_ok && (unsigned)l < (unsigned)win) {
                         if (qok) {
-                            atomicAdd(&ws[l].x, fix_rn(cr * scale));
-                            atomicAdd(&ws[l].y, fix_rn(ci * scale));
+                            atomicAdd(&ws[l], pack_fix(fix_rn(cr * scale), fix_rn(ci * scale)));
                         }
                     } else if (qok) {
                         const int nb = bin_index(k, fl, 1);
@@ -566,7 +588,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     const int NR = k.num_regions;
     for (int e = tid; e < np * win; e += kThreads) {
         const int qq = e / win, l = e - qq * win;
-        const int2 v = wsum[qq * win + l];
+        const int2 v = unpack_fix(wsum[qq * wst + l]);
         const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
         k.partial[((size_t)(j0 + qq) * NR + region) * win + l] = f;
         atomicMax(&wmax[qq], __float_as_int(fmaxf(fabsf(f.x), fabsf(f.y))));
@@ -766,12 +788,21 @@ template <int I, bool MO>
 static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const int nr8 = (k.num_regions + 7) / 8 * 8;
     const dim3 grid((unsigned)(nr8 * k.fwd_groups));
-    const size_t lds = sizeof(int2) * (size_t)k.fwd_pulses * k.win;
+    const size_t lds = sizeof(unsigned long long) * (size_t)k.fwd_pulses * k.win_stride;
+    region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(k.amps_perm, k.M,
+                                                        k.region_tiles * kTilePixels, k.region_l1);
+    const bool pl = k.planar;
     switch (k.precision) {
-        case kPrecSeries: fp_scatter_kernel<I, MO, kPrecSeries><<<grid, kThreads, lds, s>>>(k); break;
-        case kPrecHilo: fp_scatter_kernel<I, MO, kPrecHilo><<<grid, kThreads, lds, s>>>(k); break;
-        case kPrecFp32: fp_scatter_kernel<I, MO, kPrecFp32><<<grid, kThreads, lds, s>>>(k); break;
-        default: fp_scatter_kernel<I, MO, kPrecFp64><<<grid, kThreads, lds, s>>>(k); break;
+        case kPrecSeries:
+            if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, lds, s>>>(k);
+            else fp_scatter_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, lds, s>>>(k);
+            break;
+        case kPrecHilo:
+            if (pl) fp_scatter_kernel<I, MO, kPrecHilo, true><<<grid, kThreads, lds, s>>>(k);
+            else fp_scatter_kernel<I, MO, kPrecHilo, false><<<grid, kThreads, lds, s>>>(k);
+            break;
+        case kPrecFp32: fp_scatter_kernel<I, MO, kPrecFp32, false><<<grid, kThreads, lds, s>>>(k); break;
+        default: fp_scatter_kernel<I, MO, kPrecFp64, false><<<grid, kThreads, lds, s>>>(k); break;
     }
 }
 

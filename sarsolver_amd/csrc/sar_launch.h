@@ -40,9 +40,11 @@ struct KernelParams {
     int fwd_pulses;           // pulses (LDS windows) per forward workgroup, power of 2
     int fwd_groups;           // ceil(Pv / fwd_pulses)
     int win;                  // LDS window length per pulse (0 = no windows)
+    int win_stride;           // odd per-pulse stride of the LDS window planes (>= win)
     float2 *partial;          // [Pv][num_regions][win]
     double *wbases;           // [Pv][num_regions]
     float *wmax;              // [Pv][num_regions] max |component| of each window
+    float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
     // variant
     int interp, precision;
     bool mono;

@@ -123,6 +123,7 @@ struct sarhip_plan_s {
     float2 *d_partial = nullptr;
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
+    float *d_region_l1 = nullptr;
 
     rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
     rocfft_execution_info fft_info = nullptr;
@@ -157,6 +158,7 @@ struct sarhip_plan_s {
         dev_free(d_partial);
         dev_free(d_wbases);
         dev_free(d_wmax);
+        dev_free(d_region_l1);
         if (fft_work) (void)hipFree(fft_work);
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
         if (fft_bwd) rocfft_plan_destroy(fft_bwd);
@@ -197,9 +199,11 @@ struct sarhip_plan_s {
         k.fwd_pulses = fwd_pulses;
         k.fwd_groups = fwd_groups;
         k.win = win;
+        k.win_stride = win | 1;
         k.partial = d_partial;
         k.wbases = d_wbases;
         k.wmax = d_wmax;
+        k.region_l1 = d_region_l1;
         k.interp = interp;
         k.precision = precision;
         k.mono = mono;
@@ -497,13 +501,15 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     // Within a tile of radius r each leg's range moves by at most r (triangle
     // inequality), so |u - u(tile centre)| <= 2 r by_dx; series truncation and
     // rounding add far less than the one-bin slack.  Adjoint rows get a tail
-    // of ceil(2 shift_lo) + 2 bins (see bp_gather_kernel).
+    // of at least ceil(2 shift_lo) + 2 bins (see bp_gather_kernel).
     pl->shift_lo = 2.0 * rad_t * pl->by_dx * (1.0 + 1e-6) + 2.0;
     {
         const double tail = std::ceil(2.0 * pl->shift_lo) + 2.0;
         if (!(tail < 1e9) || (double)nw + tail > 2147483647.0 / 8.0)
             return fail(SARHIP_EINVAL, "scene tiles too large for the padded profile rows");
-        pl->ld = nw + (size_t)tail;
+        // rows start on 4 KiB boundaries: the strided batched FFT is as fast
+        // as the contiguous one only for aligned row starts
+        pl->ld = (nw + (size_t)tail + 511) / 512 * 512;
     }
     std::vector<PixRec> recs(M);
     for (size_t i = 0; i < M; ++i) {
@@ -558,10 +564,11 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
             const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
             if (wd > 1e6) continue;
             const int w = (int)wd;
-            for (int G : {32, 16}) {
+            for (int G : {64, 32, 16}) {
                 if (forced_g > 0 && G != forced_g) continue;
-                if ((size_t)G * w * sizeof(int2) > budget) continue;
-                const double cost = (double)w / R * (G == 32 ? 1.0 : 4.0);
+                if (G == 64 && forced_g != 64) continue;  // opt-in
+                if ((size_t)G * (w | 1) * sizeof(unsigned long long) > budget) continue;
+                const double cost = (double)w / R * (G >= 32 ? 1.0 : 4.0);
                 if (cost < best_cost) {
                     best_cost = cost;
                     best_r = R;
@@ -630,6 +637,7 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     const size_t nrows = std::max<size_t>(1, (size_t)pl->Pv);
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
+    HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
     if (pl->win > 0)
         HIP_TRY(hipMalloc((void **)&pl->d_partial,
                           sizeof(float2) * nrows * pl->num_regions * (size_t)pl->win));
