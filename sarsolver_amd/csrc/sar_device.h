@@ -84,6 +84,26 @@ __device__ __forceinline__ double ref_dr(const PulseConst &c, const double x, co
 __device__ __forceinline__ double sqrt1pm1_series(const double t) {
     return t * fma(t, fma(t, fma(t, fma(t, 0.02734375, -0.0390625), 0.0625), -0.125), 0.5);
 }
+// The two innermost coefficients held in registers the compiler cannot
+// rematerialise: gfx9 VOP3 has no literal operand, so with literals the
+// innermost step is a VOP2 fmac whose accumulator is re-initialised by two
+// v_mov per evaluation.  An identity DPP move (quad_perm [0,1,2,3]) yields
+// the same bits in a register the compiler treats as opaque, computed once.
+struct PolyRegs {
+    double c5, c4;
+};
+__device__ __forceinline__ double opaque_reg(const double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)b, 0xE4, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xE4, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ PolyRegs make_poly_regs() {
+    return PolyRegs{opaque_reg(0.02734375), opaque_reg(-0.0390625)};
+}
+__device__ __forceinline__ double sqrt1pm1_series(const double t, const PolyRegs &pr) {
+    return t * fma(t, fma(t, fma(t, fma(t, pr.c5, pr.c4), 0.0625), -0.125), 0.5);
+}
 __device__ __forceinline__ float sqrt1pm1_series_f(const float t) {
     return t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.02734375f, -0.0390625f), 0.0625f), -0.125f), 0.5f);
 }

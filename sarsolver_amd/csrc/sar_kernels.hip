@@ -215,7 +215,8 @@ __device__ __forceinline__ void load_pulse(const KernelParams &k, int tile, int 
 // PLANAR: every pixel offset has dz == 0 (the z terms are dropped).
 template <int PREC, bool MONO, bool PLANAR = false>
 __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseData<MONO> &d,
-                                           const Pix &q, double &u, double &th) {
+                                           const Pix &q, double &u, double &th,
+                                           const PolyRegs &pr) {
     if constexpr (PREC == kPrecFp64) {
         const double dr = ref_dr<MONO>(d.pc, q.x, q.y, q.z);
         u = dr * k.by_dx;
@@ -224,7 +225,7 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
         if constexpr (MONO) {
             const double t = PLANAR ? fma(d.s.bx, q.dx, fma(d.s.by, q.dy, d.s.c * q.d2))
                                     : fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
-            const double g = sqrt1pm1_series(t);
+            const double g = sqrt1pm1_series(t, pr);
             u = fma(d.s.ku, g, d.s.cu);
             th = fma(d.s.kt, g, d.s.ct);
         } else {
@@ -232,7 +233,7 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
             const double zr = PLANAR ? d.s.rc * q.d2 : fma(d.s.rbz, q.dz, d.s.rc * q.d2);
             const double tt = fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, zt));
             const double tr = fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, zr));
-            const double gt = sqrt1pm1_series(tt), gr = sqrt1pm1_series(tr);
+            const double gt = sqrt1pm1_series(tt, pr), gr = sqrt1pm1_series(tr, pr);
             u = fma(d.s.kut, gt, fma(d.s.kur, gr, d.s.cu));
             th = fma(d.s.ktt, gt, fma(d.s.ktr, gr, d.s.ct));
         }
@@ -318,6 +319,7 @@ __device__ __forceinline__ int bin_index(const KernelParams &k, double n, int sp
 template <int INTERP, bool MONO, int PREC, bool PLANAR>
 __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
     __shared__ double acc[2 * kPixPerThread][kThreads];
+    const PolyRegs pr = make_poly_regs();
     const int tile = blockIdx.x;
     const int tid = (int)threadIdx.x;
     const double4 X0 = k.tile_c[tile];
@@ -349,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
 #pragma unroll
         for (int m = 0; m < kPixPerThread; ++m) {
             double u, th;
-            eval_range<PREC, MONO, PLANAR>(k, d, q[m], u, th);
+            eval_range<PREC, MONO, PLANAR>(k, d, q[m], u, th, pr);
             float hr, hi;
             if constexpr (INTERP == kInterpLinear) {
                 const double fl = floor(u);
@@ -421,6 +423,30 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
 // expected) falls back to a global atomic into the profile.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int fix_rn(const float v) { return (int)rintf(v); }
+// Read-only element at a wave-uniform index, through the constant address
+// space so that the compiler may use scalar loads.
+typedef const __attribute__((address_space(4))) double *const_f64_ptr;
+static_assert(offsetof(PixRec, dx) == 0 && offsetof(PixRec, d2) == 24 &&
+                  offsetof(PixRec, x) == 32 && offsetof(PixRec, ok) == 56,
+              "PixRec field order");
+__device__ __forceinline__ PixRec load_uniform(const PixRec *p, size_t i) {
+    const const_f64_ptr d = (const_f64_ptr)(p + i);
+    PixRec r;
+    r.dx = d[0];
+    r.dy = d[1];
+    r.dz = d[2];
+    r.d2 = d[3];
+    r.x = d[4];
+    r.y = d[5];
+    r.z = d[6];
+    r.ok = d[7];
+    return r;
+}
+__device__ __forceinline__ float2 load_uniform(const float2 *p, size_t i) {
+    const double v = ((const_f64_ptr)(p + i))[0];
+    return make_float2(__uint_as_float((unsigned)__double_as_longlong(v)),
+                       __uint_as_float((unsigned)(__double_as_longlong(v) >> 32)));
+}
 // (re, im) -> im * 2^32 + re as a 64-bit two's-complement word
 __device__ __forceinline__ unsigned long long pack_fix(const int re, const int im) {
     return (unsigned long long)(((long long)im << 32) + (long long)re);
@@ -454,8 +480,8 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(const float2 *__res
     if (threadIdx.x == 0) out[blockIdx.x] = red[0];
 }
 
-template <int INTERP, bool MONO, int PREC, bool PLANAR>
-__global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams k) {
+template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT>
+__global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     // Fixed-point windows [G][wst], one 64-bit word per bin holding
     // im * 2^32 + re: a single 64-bit LDS add accumulates both components
     // (the low word's carries land in the high word, and both parts are
@@ -463,8 +489,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     // per-pulse stride wst is odd, spreading a wave's lanes (distinct pulses,
     // nearby bins) over the LDS banks.
     extern __shared__ unsigned long long wsum[];
-    __shared__ int wmax[kThreads];  // bit patterns of non-negative floats, [G]
-    const int G = k.fwd_pulses, R = k.region_tiles;
+    const int G = NT == 512 ? 64 : k.fwd_pulses, R = k.region_tiles;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
     // groups of one region are given ids congruent mod 8 and run back to back
     // on one XCD, whose L2 then serves the region's pixel records to all of them.
@@ -475,7 +500,10 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     if (region >= k.num_regions) return;
     const int tid = (int)threadIdx.x;
     const int q = tid & (G - 1);
-    const int stream = tid / G, nstreams = kThreads / G;
+    // G = 64: a wave's lanes are the 64 pulses and its pixel stream is
+    // wave-uniform, so pixel records and amplitudes come through scalar loads
+    const int stream = NT == 512 ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / G;
+    const int nstreams = NT / G;
     const int np = min(G, k.Pv - j0);
     const bool qok = q < np;
     const int j = j0 + (qok ? q : np - 1);
@@ -484,6 +512,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     const int t0 = region * R;
     const int t1 = min(k.num_tiles, t0 + R);
 
+    const PolyRegs pr = make_poly_regs();
     // region L1 norm (region_l1_kernel) -> fixed-point scale
     const float l1 = k.region_l1[region];
     const bool fixed_ok = isfinite(l1) && win > 0;
@@ -513,7 +542,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
     const double wb =
         floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - sleg * (1.0 + 1e-9) - 1e-9) * k.by_dx) - 2.0;
     const int wst = k.win_stride;
-    for (int e = tid; e < G * wst; e += kThreads) wsum[e] = 0ull;
+    for (int e = tid; e < G * wst; e += NT) wsum[e] = 0ull;
     __syncthreads();
     unsigned long long *ws = wsum + q * wst;
     float2 *row = k.prof + (size_t)p * k.nw;
@@ -524,9 +553,18 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
             const int tb = tile * kTilePixels;
             const int tn = min(kTilePixels, k.M - tb);
             for (int i = stream; i < tn; i += nstreams) {
-                const PixRec r = k.pixrec[tb + i];
-                if (r.ok == 0.0) continue;
-                const float2 a = k.amps_perm[tb + i];
+                PixRec r;
+                float2 a;
+                if constexpr (NT == 512) {
+                    // wave-uniform index: scalar loads through the constant address space
+                    r = load_uniform(k.pixrec, (size_t)(tb + i));
+                    if (r.ok == 0.0) continue;
+                    a = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + i));
+                } else {
+                    r = k.pixrec[tb + i];
+                    if (r.ok == 0.0) continue;
+                    a = k.amps_perm[tb + i];
+                }
                 Pix px;
                 px.x = r.x;
                 px.y = r.y;
@@ -537,7 +575,7 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
                 px.d2 = r.d2;
                 px.ok = true;
                 double u, th;
-                eval_range<PREC, MONO, PLANAR>(k, d, px, u, th);
+                eval_range<PREC, MONO, PLANAR>(k, d, px, u, th, pr);
                 float sn, cs;
                 double fl;
                 if constexpr (INTERP == kInterpLinear) {
@@ -583,36 +621,41 @@ __global__ __launch_bounds__(kThreads) void fp_scatter_kernel(const KernelParams
             }
         }
     }
-    if (tid < G) wmax[tid] = 0;
     __syncthreads();
+    // Partials and per-window maxima: wave w converts the windows of pulses
+    // w, w + NT/64, ... (coalesced rows) and reduces their maxima in registers.
     const int NR = k.num_regions;
-    for (int e = tid; e < np * win; e += kThreads) {
-        const int qq = e / win, l = e - qq * win;
-        const int2 v = unpack_fix(wsum[qq * wst + l]);
-        const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
-        k.partial[((size_t)(j0 + qq) * NR + region) * win + l] = f;
-        atomicMax(&wmax[qq], __float_as_int(fmaxf(fabsf(f.x), fabsf(f.y))));
+    const int lane = tid & 63, wave = tid >> 6;
+    for (int qq = wave; qq < np; qq += NT / 64) {
+        float mx = 0.f;
+        float2 *dst = k.partial + ((size_t)(j0 + qq) * NR + region) * win;
+        for (int l = lane; l < win; l += 64) {
+            const int2 v = unpack_fix(wsum[qq * wst + l]);
+            const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
+            dst[l] = f;
+            mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
+        }
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) k.wmax[(size_t)(j0 + qq) * NR + region] = mx;
     }
-    __syncthreads();
-    if (tid < np) {
-        const size_t w = (size_t)(j0 + tid) * NR + region;
-        k.wbases[w] = wb;
-        k.wmax[w] = __int_as_float(wmax[tid]);
-    }
+    if (tid < np) k.wbases[(size_t)(j0 + tid) * NR + region] = wb;
 }
 
 // One workgroup per valid pulse: merge the pulse's window partials into its
 // profile row.  Window bases and maxima are staged in LDS; the covered span of
 // bins is accumulated in LDS chunks while the partials stream through
-// coalesced loads.  The chunk accumulates in 32-bit fixed point with LDS
-// integer atomics, scaled by 2^30 / (sum over windows of the window's max
-// |component|), a bound on any bin sum of the row; if that bound is not
-// finite the chunk accumulates in float instead.  The chunk is then added into
-// the row.
-__global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
+// coalesced loads, kRedBatch independent loads per thread at a time so that
+// enough bytes are in flight to approach HBM bandwidth.  The chunk accumulates
+// in 32-bit fixed point with LDS integer atomics, scaled by 2^30 / (sum over
+// windows of the window's max |component|), a bound on any bin sum of the
+// row; if that bound is not finite the chunk accumulates in float instead.
+// The chunk is then added into the row.
+constexpr int kRedThreads = 512;
+constexpr int kRedBatch = 8;
+__global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
     extern __shared__ int2 acc[];  // [chunk], then [NR] bases (double)
-    __shared__ double red_lo[kThreads], red_hi[kThreads];
-    __shared__ float red_l1[kThreads];
+    __shared__ double red_lo[kRedThreads], red_hi[kRedThreads];
+    __shared__ float red_l1[kRedThreads];
     const int j = blockIdx.x;
     const int p = k.rows[j];
     const int NR = k.num_regions, win = k.win;
@@ -623,7 +666,7 @@ __global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams 
     const int tid = (int)threadIdx.x;
     double lo = INFINITY, hi = -INFINITY;
     float l1 = 0.f;
-    for (int t = tid; t < NR; t += kThreads) {
+    for (int t = tid; t < NR; t += kRedThreads) {
         const double bt = b[t];
         sb[t] = bt;
         lo = fmin(lo, bt);
@@ -634,7 +677,7 @@ __global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams 
     red_hi[tid] = hi;
     red_l1[tid] = l1;
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
+    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
         if (tid < s) {
             red_lo[tid] = fmin(red_lo[tid], red_lo[tid + s]);
             red_hi[tid] = fmax(red_hi[tid], red_hi[tid + s]);
@@ -651,36 +694,47 @@ __global__ __launch_bounds__(kThreads) void fp_reduce_kernel(const KernelParams 
     if (fixed && scale == 0.f) return;  // every window is zero: the row stays zero
     float2 *facc = (float2 *)acc;
     float2 *row = k.prof + (size_t)p * k.nw;
-    // thread walks elements e = tid + 256 i of [NR][win] without dividing
-    const int t_step = kThreads / win, l_step = kThreads % win;
+    // thread walks elements e = tid + kRedThreads i of [NR][win] without dividing
+    const int t_step = kRedThreads / win, l_step = kRedThreads % win;
+    const size_t total = (size_t)NR * win;
     for (double c0 = lo; c0 < hi; c0 += (double)chunk) {
-        for (int e = tid; e < chunk; e += kThreads) acc[e] = make_int2(0, 0);
+        for (int e = tid; e < chunk; e += kRedThreads) acc[e] = make_int2(0, 0);
         __syncthreads();
         int t = tid / win, l = tid % win;
-        const size_t total = (size_t)NR * win;
-        for (size_t e = tid; e < total; e += kThreads) {
-            const float2 v = part[e];
-            const double rel = sb[t] + (double)l - c0;
-            if (rel >= 0.0 && rel < (double)chunk) {
-                const int r = (int)rel;
-                if (fixed) {
-                    atomicAdd(&acc[r].x, (int)rintf(v.x * scale));
-                    atomicAdd(&acc[r].y, (int)rintf(v.y * scale));
-                } else {
-                    atomicAdd(&facc[r].x, v.x);
-                    atomicAdd(&facc[r].y, v.y);
-                }
+        for (size_t e0 = tid; e0 < total; e0 += (size_t)kRedThreads * kRedBatch) {
+            float2 v[kRedBatch];
+#pragma unroll
+            for (int i = 0; i < kRedBatch; ++i) {
+                const size_t e = e0 + (size_t)i * kRedThreads;
+                v[i] = e < total ? part[e] : make_float2(0.f, 0.f);
             }
-            t += t_step;
-            l += l_step;
-            if (l >= win) {
-                l -= win;
-                t += 1;
+#pragma unroll
+            for (int i = 0; i < kRedBatch; ++i) {
+                const size_t e = e0 + (size_t)i * kRedThreads;
+                if (e < total) {
+                    const double rel = sb[t] + (double)l - c0;
+                    if (rel >= 0.0 && rel < (double)chunk) {
+                        const int r = (int)rel;
+                        if (fixed) {
+                            atomicAdd(&acc[r].x, (int)rintf(v[i].x * scale));
+                            atomicAdd(&acc[r].y, (int)rintf(v[i].y * scale));
+                        } else {
+                            atomicAdd(&facc[r].x, v[i].x);
+                            atomicAdd(&facc[r].y, v[i].y);
+                        }
+                    }
+                }
+                t += t_step;
+                l += l_step;
+                if (l >= win) {
+                    l -= win;
+                    t += 1;
+                }
             }
         }
         __syncthreads();
         const int len = (int)fmin((double)chunk, hi - c0);
-        for (int e = tid; e < len; e += kThreads) {
+        for (int e = tid; e < len; e += kRedThreads) {
             const int n = bin_index(k, c0 + (double)e, 1);
             float2 r = row[n];
             if (fixed) {
@@ -784,43 +838,50 @@ hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <int I, bool MO>
+template <int I, bool MO, int NT>
 static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const int nr8 = (k.num_regions + 7) / 8 * 8;
     const dim3 grid((unsigned)(nr8 * k.fwd_groups));
     const size_t lds = sizeof(unsigned long long) * (size_t)k.fwd_pulses * k.win_stride;
-    region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(k.amps_perm, k.M,
-                                                        k.region_tiles * kTilePixels, k.region_l1);
     const bool pl = k.planar;
     switch (k.precision) {
         case kPrecSeries:
-            if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, lds, s>>>(k);
-            else fp_scatter_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, lds, s>>>(k);
+            if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true, NT><<<grid, NT, lds, s>>>(k);
+            else fp_scatter_kernel<I, MO, kPrecSeries, false, NT><<<grid, NT, lds, s>>>(k);
             break;
         case kPrecHilo:
-            if (pl) fp_scatter_kernel<I, MO, kPrecHilo, true><<<grid, kThreads, lds, s>>>(k);
-            else fp_scatter_kernel<I, MO, kPrecHilo, false><<<grid, kThreads, lds, s>>>(k);
+            if (pl) fp_scatter_kernel<I, MO, kPrecHilo, true, NT><<<grid, NT, lds, s>>>(k);
+            else fp_scatter_kernel<I, MO, kPrecHilo, false, NT><<<grid, NT, lds, s>>>(k);
             break;
-        case kPrecFp32: fp_scatter_kernel<I, MO, kPrecFp32, false><<<grid, kThreads, lds, s>>>(k); break;
-        default: fp_scatter_kernel<I, MO, kPrecFp64, false><<<grid, kThreads, lds, s>>>(k); break;
+        case kPrecFp32: fp_scatter_kernel<I, MO, kPrecFp32, false, NT><<<grid, NT, lds, s>>>(k); break;
+        default: fp_scatter_kernel<I, MO, kPrecFp64, false, NT><<<grid, NT, lds, s>>>(k); break;
     }
+}
+
+// 64 pulses per workgroup run with 512 threads (8 pixel streams), fewer with 256.
+template <int I, bool MO>
+static void fp_dispatch_threads(const KernelParams &k, hipStream_t s) {
+    region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(k.amps_perm, k.M,
+                                                        k.region_tiles * kTilePixels, k.region_l1);
+    if (k.fwd_pulses == 64) fp_dispatch_prec<I, MO, 512>(k, s);
+    else fp_dispatch_prec<I, MO, kThreads>(k, s);
 }
 
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s) {
     // nearest_ref and nearest share the forward bin rule round(u) (CPP:289)
     if (k.mono) {
-        if (k.interp == kInterpLinear) fp_dispatch_prec<kInterpLinear, true>(k, s);
-        else fp_dispatch_prec<kInterpNearest, true>(k, s);
+        if (k.interp == kInterpLinear) fp_dispatch_threads<kInterpLinear, true>(k, s);
+        else fp_dispatch_threads<kInterpNearest, true>(k, s);
     } else {
-        if (k.interp == kInterpLinear) fp_dispatch_prec<kInterpLinear, false>(k, s);
-        else fp_dispatch_prec<kInterpNearest, false>(k, s);
+        if (k.interp == kInterpLinear) fp_dispatch_threads<kInterpLinear, false>(k, s);
+        else fp_dispatch_threads<kInterpNearest, false>(k, s);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s) {
     const size_t lds = sizeof(int2) * (size_t)chunk + sizeof(double) * (size_t)k.num_regions;
-    fp_reduce_kernel<<<k.Pv, kThreads, lds, s>>>(k, chunk);
+    fp_reduce_kernel<<<k.Pv, kRedThreads, lds, s>>>(k, chunk);
     return hipGetLastError();
 }
 

@@ -556,26 +556,33 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         const int forced_g = env_int("SARHIP_FP_PULSES", 0);
         double best_cost = INFINITY;
         int best_r = 1, best_g = 32, best_w = 0;
-        for (int R : {1, 2, 4, 8, 16}) {
-            if (forced_r > 0 && R != forced_r) continue;
-            std::vector<double4> reg, hal;
-            double rad = 0.0;
-            make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad, &hal);
-            const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
-            if (wd > 1e6) continue;
-            const int w = (int)wd;
-            for (int G : {64, 32, 16}) {
-                if (forced_g > 0 && G != forced_g) continue;
-                if (G == 64 && forced_g != 64) continue;  // opt-in
-                if ((size_t)G * (w | 1) * sizeof(unsigned long long) > budget) continue;
-                const double cost = (double)w / R * (G >= 32 ? 1.0 : 4.0);
-                if (cost < best_cost) {
-                    best_cost = cost;
-                    best_r = R;
-                    best_g = G;
-                    best_w = w;
+        // 64 pulses per workgroup (512 threads, wave-uniform pixel streams)
+        // whenever its windows fit in 64 KiB; otherwise 32 or 16 pulses under
+        // `budget`.  Among region sizes, fewest window bins per tile.
+        const size_t budget64 = 64 * 1024;
+        for (int pass = 0; pass < 2 && !std::isfinite(best_cost); ++pass) {
+            for (int R : {1, 2, 4, 8, 16}) {
+                if (forced_r > 0 && R != forced_r) continue;
+                std::vector<double4> reg, hal;
+                double rad = 0.0;
+                make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad, &hal);
+                const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
+                if (wd > 1e6) continue;
+                const int w = (int)wd;
+                for (int G : {64, 32, 16}) {
+                    if (forced_g > 0 && G != forced_g) continue;
+                    if ((pass == 0) != (G == 64)) continue;
+                    const size_t lim = G == 64 ? budget64 : budget;
+                    if ((size_t)G * (w | 1) * sizeof(unsigned long long) > lim) continue;
+                    const double cost = (double)w / R * (G >= 32 ? 1.0 : 4.0);
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_r = R;
+                        best_g = G;
+                        best_w = w;
+                    }
+                    break;  // the larger G that fits is preferred for this R
                 }
-                break;  // the larger G that fits is preferred for this R
             }
         }
         if (env_int("SARHIP_FP_NO_WINDOWS", 0)) best_cost = INFINITY;
