@@ -552,14 +552,27 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             load_pulse<PREC, MONO>(k, tile, j, p, d);
             const int tb = tile * kTilePixels;
             const int tn = min(kTilePixels, k.M - tb);
+            // NT == 512: the next pixel's record is requested one iteration ahead
+            PixRec r_next;
+            float2 a_next;
+            if constexpr (NT == 512) {
+                if (stream < tn) {
+                    r_next = load_uniform(k.pixrec, (size_t)(tb + stream));
+                    a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + stream));
+                }
+            }
             for (int i = stream; i < tn; i += nstreams) {
                 PixRec r;
                 float2 a;
                 if constexpr (NT == 512) {
                     // wave-uniform index: scalar loads through the constant address space
-                    r = load_uniform(k.pixrec, (size_t)(tb + i));
+                    r = r_next;
+                    a = a_next;
+                    if (i + nstreams < tn) {
+                        r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
+                        a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + i + nstreams));
+                    }
                     if (r.ok == 0.0) continue;
-                    a = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + i));
                 } else {
                     r = k.pixrec[tb + i];
                     if (r.ok == 0.0) continue;
