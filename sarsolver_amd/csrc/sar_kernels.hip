@@ -422,7 +422,18 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
 // does not depend on scheduling.  A contribution outside its window (not
 // expected) falls back to a global atomic into the profile.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int fix_rn(const float v) { return (int)rintf(v); }
+// Nearest integer, halves rounded up: floor(v + 0.5) in one v_cvt_rpi_i32_f32
+// (|v| < 2^31 on every use; the error stays within one unit).
+__device__ __forceinline__ int fix_rn(const float v) {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+// Fixed-point scale of a region with L1 norm l1: 2^30 / (1.001 l1), so that no
+// bin sum of the region reaches 2^30 (interpolation weights are <= 1).
+__device__ __forceinline__ float fixed_scale(const float l1) {
+    return 1073741824.0f / (l1 * 1.001f);
+}
 // Read-only element at a wave-uniform index, through the constant address
 // space so that the compiler may use scalar loads.
 typedef const __attribute__((address_space(4))) double *const_f64_ptr;
@@ -459,9 +470,12 @@ __device__ __forceinline__ int2 unpack_fix(const unsigned long long v) {
 }
 
 // Per-region L1 norm sum(|Re a| + |Im a|) of the permuted amplitudes (FP32,
-// fixed reduction order), for the forward's fixed-point scale.
-__global__ __launch_bounds__(kThreads) void region_l1_kernel(const float2 *__restrict__ amps,
+// fixed reduction order), and the amplitudes scaled in place by the region's
+// fixed-point scale (when windows are used and the norm is finite and
+// non-zero), so the scatter forms fixed-point contributions without a multiply.
+__global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict__ amps,
                                                              int M, int region_pixels,
+                                                             int windows,
                                                              float *__restrict__ out) {
     __shared__ float red[kThreads];
     const int base = blockIdx.x * region_pixels;
@@ -477,7 +491,15 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(const float2 *__res
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+    const float tot = red[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = tot;
+    if (windows && isfinite(tot) && tot > 0.f) {
+        const float sc = fixed_scale(tot);
+        for (int i = threadIdx.x; i < n; i += kThreads) {
+            const float2 a = amps[base + i];
+            amps[base + i] = make_float2(a.x * sc, a.y * sc);
+        }
+    }
 }
 
 template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT>
@@ -516,8 +538,12 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     // region L1 norm (region_l1_kernel) -> fixed-point scale
     const float l1 = k.region_l1[region];
     const bool fixed_ok = isfinite(l1) && win > 0;
-    const float scale = (fixed_ok && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
+    const float scale = (fixed_ok && l1 > 0.f) ? fixed_scale(l1) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
+    // amplitudes arrive pre-scaled by `scale` when it is non-zero
+    const float unscale = scale > 0.f ? inv_scale : 1.0f;
+    // linear series / hilo: u is evaluated relative to the window base
+    constexpr bool kRel = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
 
     // Window base: for a leg with phase centre T, unit vector e from the region
     // centre Xc to T and a pixel X = Xc + d in the region's box of half-extents h,
@@ -550,6 +576,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
         for (int tile = t0; tile < t1; ++tile) {
             PulseData<MONO> d;
             load_pulse<PREC, MONO>(k, tile, j, p, d);
+            if constexpr (kRel) d.s.cu -= wb;
             const int tb = tile * kTilePixels;
             const int tn = min(kTilePixels, k.M - tb);
             // NT == 512: the next pixel's record is requested one iteration ahead
@@ -598,37 +625,35 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
                 }
                 phasor_rev(th, &sn, &cs);
+                // contribution a e^{i theta} (times `scale` when pre-scaled)
                 const float cr = fmaf(cs, a.x, -(sn * a.y));
                 const float ci = fmaf(cs, a.y, sn * a.x);
-                const int l = (int)(fl - wb);
+                const int l = kRel ? (int)fl : (int)(fl - wb);
                 if constexpr (INTERP == kInterpLinear) {
                     const float fr = (float)(u - fl);
                     if (fixed_ok && (unsigned)l < (unsigned)(win - 1)) {
                         if (qok) {
-                            const float sr = cr * scale, si = ci * scale;
-                            const int tr = fix_rn(sr), ti = fix_rn(si);
-                            const int r1 = fix_rn(fr * sr), i1 = fix_rn(fr * si);
+                            const int tr = fix_rn(cr), ti = fix_rn(ci);
+                            const int r1 = fix_rn(fr * cr), i1 = fix_rn(fr * ci);
                             atomicAdd(&ws[l], pack_fix(tr - r1, ti - i1));
                             atomicAdd(&ws[l + 1], pack_fix(r1, i1));
                         }
                     } else if (qok) {
-                        const float gr = 1.0f - fr;
-                        const int n0 = bin_index(k, fl, 1);
+                        const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
+                        const int n0 = bin_index(k, kRel ? fl + wb : fl, 1);
                         const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
                         atomicAdd(&row[n0].x, gr * cr);
                         atomicAdd(&row[n0].y, gr * ci);
-                        atomicAdd(&row[n1].x, fr * cr);
-                        atomicAdd(&row[n1].y, fr * ci);
+                        atomicAdd(&row[n1].x, fu * cr);
+                        atomicAdd(&row[n1].y, fu * ci);
                     }
                 } else {
                     if (fixed_ok && (unsigned)l < (unsigned)win) {
-                        if (qok) {
-                            atomicAdd(&ws[l], pack_fix(fix_rn(cr * scale), fix_rn(ci * scale)));
-                        }
+                        if (qok) atomicAdd(&ws[l], pack_fix(fix_rn(cr), fix_rn(ci)));
                     } else if (qok) {
                         const int nb = bin_index(k, fl, 1);
-                        atomicAdd(&row[nb].x, cr);
-                        atomicAdd(&row[nb].y, ci);
+                        atomicAdd(&row[nb].x, cr * unscale);
+                        atomicAdd(&row[nb].y, ci * unscale);
                     }
                 }
             }
@@ -874,8 +899,9 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
 // 64 pulses per workgroup run with 512 threads (8 pixel streams), fewer with 256.
 template <int I, bool MO>
 static void fp_dispatch_threads(const KernelParams &k, hipStream_t s) {
-    region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(k.amps_perm, k.M,
-                                                        k.region_tiles * kTilePixels, k.region_l1);
+    region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(const_cast<float2 *>(k.amps_perm), k.M,
+                                                        k.region_tiles * kTilePixels, k.win > 0,
+                                                        k.region_l1);
     if (k.fwd_pulses == 64) fp_dispatch_prec<I, MO, 512>(k, s);
     else fp_dispatch_prec<I, MO, kThreads>(k, s);
 }

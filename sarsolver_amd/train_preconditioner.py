@@ -56,6 +56,10 @@ def main(argv=None) -> int:
     ap.add_argument("--lr", type=float, default=1e-4)
     ap.add_argument("--eval-scenes", type=int, default=64)
     ap.add_argument("--summary", default=None, help="also write the summary JSON here")
+    ap.add_argument("--init", default=None,
+                    help="continue training from these weights (the circulant base keeps their tau)")
+    ap.add_argument("--seed", type=int, default=5, help="seed of the training scenes")
+    ap.add_argument("--no-eval", action="store_true", help="skip the held-out evaluation")
     a = ap.parse_args(argv)
     if not torch.cuda.is_available():
         print("a GPU is required", file=sys.stderr)
@@ -63,38 +67,46 @@ def main(argv=None) -> int:
     dev = torch.device("cuda", 0)
     w, op = workload_operator(a.workload, dev)
     n, lam = w.n, 1e-3
-    gens = learned.make_gens(777, dev)
-    with torch.no_grad():
-        probs = [learned.noisy_problem(op, x, 20.0, gens.dev_gen)
-                 for x in learned.random_scenes(n, 8, gens, dev)]
-    tau, scores = learned.choose_tau(op, n, TAUS, probs, lam=lam, pad=PAD_FACTOR * n)
-    print(f"tau scores (median PCG iterations): {scores} -> tau {tau}", flush=True)
-    base = learned.CirculantPreconditioner(op, n, tau, pad=PAD_FACTOR * n)
-    pre = learned.CnnPreconditioner(dev, base=base)
+    if a.init:
+        pre = learned.load_preconditioner(a.init, op, n)
+        base, tau = pre.base, pre.base.tau
+        print(f"continuing from {a.init} (tau {tau})", flush=True)
+    else:
+        gens = learned.make_gens(777, dev)
+        with torch.no_grad():
+            probs = [learned.noisy_problem(op, x, 20.0, gens.dev_gen)
+                     for x in learned.random_scenes(n, 8, gens, dev)]
+        tau, scores = learned.choose_tau(op, n, TAUS, probs, lam=lam, pad=PAD_FACTOR * n)
+        print(f"tau scores (median PCG iterations): {scores} -> tau {tau}", flush=True)
+        base = learned.CirculantPreconditioner(op, n, tau, pad=PAD_FACTOR * n)
+        pre = learned.CnnPreconditioner(dev, base=base)
     cval = learned.problem_batch(op, n, 4, learned.make_gens(4242, dev))
     with torch.no_grad():
+        lb = float(learned.unrolled_pcg_loss(op, base, cval, lam, a.iters))
         l0 = float(learned.unrolled_pcg_loss(op, lambda r: pre(r, (n, n)), cval, lam, a.iters))
     prog = lambda st, l, el: print(f"  step {st} loss {l:.4f} ({el:.1f} s)", flush=True)
     pre, info = learned.train_unrolled(op, pre, n, lam=lam, iters=a.iters, steps=a.steps,
-                                       batch=a.batch, lr=a.lr, log_every=max(1, a.steps // 10),
-                                       progress=prog)
+                                       batch=a.batch, lr=a.lr, seed=a.seed,
+                                       log_every=max(1, a.steps // 20), progress=prog)
     with torch.no_grad():
         l1 = float(learned.unrolled_pcg_loss(op, lambda r: pre(r, (n, n)), cval, lam, a.iters))
-    print(f"held-out unrolled loss (mean log10 residual over {a.iters} its): "
-          f"base {l0:.4f} -> learned {l1:.4f}", flush=True)
+    print(f"held-out unrolled loss (mean log10 residual over {a.iters} its): circulant "
+          f"base alone {lb:.4f}; network before {l0:.4f} -> after {l1:.4f}", flush=True)
     out = Path(a.out) if a.out else default_weights(a.workload)
     out.parent.mkdir(parents=True, exist_ok=True)
     learned.save_preconditioner(out, pre, {
         "tau": tau, "pad": PAD_FACTOR * n, "steps": a.steps, "iters": a.iters, "lr": a.lr,
-        "batch": a.batch, "lam": lam, "train_seconds": info["seconds"],
-        "loss_base": l0, "loss_learned": l1})
+        "batch": a.batch, "seed": a.seed, "lam": lam, "train_seconds": info["seconds"],
+        "loss_base": lb, "loss_start": l0, "loss_learned": l1})
     print(f"weights -> {out}", flush=True)
+    if a.no_eval:
+        return 0
     t = time.perf_counter()
     res = learned.evaluate(op, {"circulant": base, "learned": lambda r: pre(r, (n, n))}, n,
                            scenes=a.eval_scenes, lam=lam)
     summary = {k: v for k, v in res.items() if k != "rows"}
     summary.update({"workload": a.workload, "tau": tau, "train_seconds": info["seconds"],
-                    "loss_base": l0, "loss_learned": l1,
+                    "loss_base": lb, "loss_start": l0, "loss_learned": l1,
                     "eval_seconds": time.perf_counter() - t})
     print(json.dumps(summary), flush=True)
     if a.summary:
