@@ -45,6 +45,8 @@ struct KernelParams {
     double *wbases;           // [Pv][num_regions]
     float *wmax;              // [Pv][num_regions] max |component| of each window
     float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
+    int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
+    double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
     // variant
     int interp, precision;
     bool mono;

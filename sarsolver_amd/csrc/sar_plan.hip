@@ -124,6 +124,8 @@ struct sarhip_plan_s {
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
     float *d_region_l1 = nullptr;
+    double2 *d_bp_partial = nullptr;
+    int bp_splits = 1;
 
     rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
     rocfft_execution_info fft_info = nullptr;
@@ -159,6 +161,7 @@ struct sarhip_plan_s {
         dev_free(d_wbases);
         dev_free(d_wmax);
         dev_free(d_region_l1);
+        dev_free(d_bp_partial);
         if (fft_work) (void)hipFree(fft_work);
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
         if (fft_bwd) rocfft_plan_destroy(fft_bwd);
@@ -204,6 +207,8 @@ struct sarhip_plan_s {
         k.wbases = d_wbases;
         k.wmax = d_wmax;
         k.region_l1 = d_region_l1;
+        k.bp_splits = bp_splits;
+        k.bp_partial = d_bp_partial;
         k.interp = interp;
         k.precision = precision;
         k.mono = mono;
@@ -645,6 +650,17 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
+    // Backprojection: split each tile's pulses over several workgroups until
+    // the grid has about 8 workgroups per CU (occupancy), keeping at least 128
+    // pulses per split; partial images are summed in FP64 in split order.
+    {
+        int S = 1;
+        while ((long long)pl->num_tiles * S < 2048 && pl->Pv / (2 * S) >= 128) S *= 2;
+        const int forced = env_int("SARHIP_BP_SPLITS", 0);
+        if (forced > 0) S = std::min(forced, std::max(pl->Pv, 1));
+        pl->bp_splits = S;
+        if (S > 1) HIP_TRY(hipMalloc((void **)&pl->d_bp_partial, sizeof(double2) * S * M));
+    }
     if (pl->win > 0)
         HIP_TRY(hipMalloc((void **)&pl->d_partial,
                           sizeof(float2) * nrows * pl->num_regions * (size_t)pl->win));

@@ -2,17 +2,21 @@
 
 Operators are any objects with `forward(x)` and `adjoint(y)` on torch tensors
 (for instance `torch_ops.SarOperator`).  Vectors stay on the operator's device;
-each iteration synchronises once to read the residual norm.
+each iteration synchronises with the host once, to read the scalars that
+decide whether to stop (both solvers are written the same way, so their
+wall times compare the methods and not the bookkeeping).
 
 * `cgls`: conjugate gradients on the normal equations written with A and A^H
   (one A and one A^H per iteration).  It is the fallback and the judge: every
   solve is scored by `true_residual`, recomputed from scratch.
 * `fpcg`: flexible PCG with a preconditioner callable z = M(r) that may be
-  nonlinear (a trained network): Polak-Ribiere beta.  Non-finite values or
+  nonlinear (a trained network): Polak-Ribiere beta.  Non-finite values,
   non-positive curvature p^H K p <= 0, or a residual grown to 10x the
   smallest seen so far, switch the solve to CGLS from the best iterate
   (reported in the result).
 Stopping test: ||A^H b - (A^H A + lam I) x|| / ||A^H b|| <= tol.
+Inner products accumulate in the vectors' precision (complex64 on the GPU);
+`true_residual` recomputes the residual of a result in FP64.
 """
 from __future__ import annotations
 
@@ -25,15 +29,21 @@ import torch
 
 
 def _dot(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
-    """a^H b accumulated in float64 / complex128."""
+    """a^H b as a 0-d tensor (no host synchronisation)."""
     if a.is_complex():
-        return torch.vdot(a.reshape(-1).to(torch.complex128), b.reshape(-1).to(torch.complex128))
-    return torch.dot(a.reshape(-1).double(), b.reshape(-1).double())
+        return torch.vdot(a.reshape(-1), b.reshape(-1))
+    return torch.dot(a.reshape(-1), b.reshape(-1))
 
 
 def _norm(a: torch.Tensor) -> float:
+    """||a|| accumulated in FP64 (synchronises)."""
     return float(torch.linalg.vector_norm(a.reshape(-1).to(
         torch.complex128 if a.is_complex() else torch.float64)))
+
+
+def _host(*ts: torch.Tensor) -> List[float]:
+    """Real parts of several 0-d tensors in one device-to-host transfer."""
+    return torch.stack([t.real if t.is_complex() else t for t in ts]).double().tolist()
 
 
 @dataclass
@@ -69,30 +79,31 @@ def cgls(op, b: torch.Tensor, lam: float = 0.0, tol: float = 1e-3, max_iter: int
     if x0 is None:
         x = torch.zeros_like(c)
         r = b.clone()
-        s = c.clone()
+        s = c
     else:
         x = x0.clone()
         r = b - op.forward(x)
         s = op.adjoint(r) - lam * x
-    p = s.clone()
+    p = s
     gamma = _dot(s, s).real
-    res = [math.sqrt(float(gamma)) / c_norm]
+    res = [math.sqrt(max(_host(gamma)[0], 0.0)) / c_norm]
     it = 0
     converged = res[-1] <= tol
     while not converged and it < max_iter:
         q = op.forward(p)
         delta = _dot(q, q).real + lam * _dot(p, p).real
-        if not (float(delta) > 0.0):
-            break
         alpha = (gamma / delta).to(x.dtype)
+        x_prev = x
         x = x + alpha * p
         r = r - alpha * q
         s = op.adjoint(r) - lam * x
         gamma_new = _dot(s, s).real
-        it += 1
-        res.append(math.sqrt(float(gamma_new)) / c_norm)
-        if not math.isfinite(res[-1]):
+        d, g = _host(delta, gamma_new)
+        if not (d > 0.0) or not math.isfinite(d) or not math.isfinite(g):
+            x = x_prev  # the step was not defined: keep the last good iterate
             break
+        it += 1
+        res.append(math.sqrt(max(g, 0.0)) / c_norm)
         converged = res[-1] <= tol
         p = s + (gamma_new / gamma).to(x.dtype) * p
         gamma = gamma_new
@@ -109,9 +120,9 @@ def fpcg(op, b: torch.Tensor, precond: Callable[[torch.Tensor], torch.Tensor], l
     c = op.adjoint(b)
     c_norm = max(_norm(c), 1e-300)
     x = torch.zeros_like(c)
-    r = c.clone()
+    r = c
     z = precond(r)
-    p = z.clone()
+    p = z
     rz = _dot(r, z)
     res = [1.0]
     it = 0
@@ -119,36 +130,34 @@ def fpcg(op, b: torch.Tensor, precond: Callable[[torch.Tensor], torch.Tensor], l
     converged = False
     best_x, best_res = x, 1.0
     while it < max_iter:
-        if not bool(torch.isfinite(torch.view_as_real(z) if z.is_complex() else z).all()):
-            fallback = "preconditioner returned non-finite values"
-            x = best_x
-            break
         Kp = normal_apply(op, p, lam)
         pKp = _dot(p, Kp)
-        if not (float(pKp.real) > 0.0) or not math.isfinite(float(pKp.real)):
-            fallback = "non-positive curvature p^H K p <= 0"
+        alpha = (rz / pKp).to(x.dtype)
+        x_new = x + alpha * p
+        r_new = r - alpha * Kp
+        rr = _dot(r_new, r_new).real
+        curv, rr_h = _host(pKp, rr)
+        # a non-finite z (preconditioner) or step shows up here as well
+        if not (curv > 0.0) or not math.isfinite(curv) or not math.isfinite(rr_h):
+            fallback = "non-positive or non-finite curvature p^H K p"
             x = best_x
             break
-        alpha = (rz / pKp).to(x.dtype)
-        x = x + alpha * p
-        r_new = r - alpha * Kp
-        it += 1
-        res.append(_norm(r_new) / c_norm)
+        x, it = x_new, it + 1
+        res.append(math.sqrt(max(rr_h, 0.0)) / c_norm)
         if res[-1] <= tol:
             converged = True
-            r = r_new
             break
         if res[-1] < best_res:
             best_x, best_res = x, res[-1]
-        elif not math.isfinite(res[-1]) or res[-1] > 10.0 * best_res:
+        elif res[-1] > 10.0 * best_res:
             fallback = "residual grew to 10x its minimum"
             x = best_x
             break
         z_new = precond(r_new)
         rz_new = _dot(r_new, z_new)
-        beta = (_dot(r_new - r, z_new) / rz).to(x.dtype)  # Polak-Ribiere
+        beta = ((rz_new - _dot(r, z_new)) / rz).to(x.dtype)  # Polak-Ribiere
         p = z_new + beta * p
-        r, z, rz = r_new, z_new, rz_new
+        r, rz = r_new, rz_new
     if fallback is not None:
         rest = cgls(op, b, lam=lam, tol=tol, max_iter=max(1, max_iter - it), x0=x,
                     sync_time=False)
