@@ -564,7 +564,8 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         // 64 pulses per workgroup (512 threads, wave-uniform pixel streams)
         // whenever its windows fit in 64 KiB; otherwise 32 or 16 pulses under
         // `budget`.  Among region sizes, fewest window bins per tile.
-        const size_t budget64 = 64 * 1024;
+        const size_t budget64 =
+            (size_t)std::min(std::max(env_int("SARHIP_FP_LDS64_KB", 64), 16), 160) * 1024;
         for (int pass = 0; pass < 2 && !std::isfinite(best_cost); ++pass) {
             for (int R : {1, 2, 4, 8, 16}) {
                 if (forced_r > 0 && R != forced_r) continue;
