@@ -153,6 +153,25 @@ const char *sarhip_last_error(void);
 /* Library version string. */
 const char *sarhip_version(void);
 
+/* Dataset resampling, replacing the window loop of
+ * NumpySimpleSarDataset.from_resampling (numpy/measurement.py:397-457), which
+ * calls move_srps (utils/geodesy.py:33-73), fast_freq_downsample
+ * (utils/signal.py:45-73) and the slow-time kernel of get_downsample_kernel
+ * (utils/signal.py:7-25).  Output pulse i is
+ *   sum_k kern[k] * fast_freq_downsample(move_srps(ph_in[starts[i] + k]))
+ * with move_srps's phase exp(2 pi i dr (f_bb - sign fc) / c) per input pulse
+ * (dr = new minus old bistatic SRP range, f_bb the baseband frequency of the
+ * sample).  ph_in [P][N] and ph_out [P2][N2] are device complex64 arrays in
+ * centred frequency order; dr [P], win [N2] (= fftshift(tukey(N2)) / N),
+ * kern [K] (complex128 interleaved) and starts [P2] are host arrays.  Work
+ * buffers and FFT plans are kept per device for the last (P, N, N2) shape
+ * (held until the process ends); the call returns once `stream` has finished.
+ * Calls from several host threads are serialised. */
+int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, double centre_freq,
+                    double sample_freq, double sign_multiplier, double c_eff, size_t N2,
+                    const double *win, const double *kern, int K, const int *starts, size_t P2,
+                    void *ph_out, void *stream);
+
 /* ---- 3x3 convolutions of the learned CG preconditioner ----------------------
  * No counterpart in the reference (it has no learned components); these
  * serve the preconditioner network that accelerates CG on the normal

@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
     "sarhip_plan_set_timing",
     "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
+    "sarhip_resample",
     "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
     "sarhip_conv_last_error",
 )
@@ -103,6 +104,10 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_last_error.restype = ctypes.c_char_p
         L.sarhip_version.argtypes = []
         L.sarhip_version.restype = ctypes.c_char_p
+        sz, dbl = ctypes.c_size_t, ctypes.c_double
+        L.sarhip_resample.argtypes = [vp, sz, sz, dp, dbl, dbl, dbl, dbl, sz, dp, dp,
+                                      ctypes.c_int, ip, sz, vp, vp]
+        L.sarhip_resample.restype = ctypes.c_int
         _lib = L
         return L
 
@@ -276,6 +281,36 @@ class SarPlan:
                                                 buf, ctypes.byref(n)), "sarhip_plan_timing")
         return {"window": buf[0], "fft": buf[1], "kernel": buf[2], "total": buf[3],
                 "calls": n.value}
+
+
+def resample(phase_history, dr, centre_freq: float, sample_freq: float, sign_multiplier: float,
+             c_eff: float, new_num_freqs: int, window, kernel, starts):
+    """Device resampling (`sarhip_resample`): phase_history [P, N] complex64
+    torch tensor (centred order) on a GPU -> [len(starts), new_num_freqs]
+    complex64 tensor on the same device.  dr [P], window [new_num_freqs] (FFT
+    order, including 1/N), kernel [K] complex and starts [P2] are host arrays."""
+    torch = _torch()
+    L = load_library()
+    if phase_history.dtype != torch.complex64 or not phase_history.is_cuda:
+        raise TypeError("phase_history must be a complex64 tensor on a GPU")
+    ph = phase_history.contiguous()
+    P, N = ph.shape
+    drh = _f64(dr).reshape(-1)
+    win = _f64(window).reshape(-1)
+    kern = _c128_interleaved(kernel)
+    st = np.ascontiguousarray(np.asarray(starts, dtype=np.int32).reshape(-1))
+    if drh.shape[0] != P or win.shape[0] != new_num_freqs:
+        raise ValueError("dr must have one entry per pulse and window one per new frequency")
+    out = torch.empty((st.shape[0], new_num_freqs), dtype=torch.complex64, device=ph.device)
+    with torch.cuda.device(ph.device):
+        rc = L.sarhip_resample(ctypes.c_void_p(ph.data_ptr()), P, N, _dptr(drh),
+                               float(centre_freq), float(sample_freq), float(sign_multiplier),
+                               float(c_eff), int(new_num_freqs), _dptr(win), _dptr(kern),
+                               kern.shape[0] // 2,
+                               st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), st.shape[0],
+                               ctypes.c_void_p(out.data_ptr()), _stream_handle(ph.device))
+    check(rc, "sarhip_resample")
+    return out
 
 
 def evaluate_host(info: SarCalculationInfo, forward: bool) -> None:

@@ -367,6 +367,128 @@ extern "C" {
 
 const char *sarhip_last_error(void) { return g_last_error.c_str(); }
 
+// Dataset resampling (numpy/measurement.py:342-457): see resample_kernels.hip.
+// Host arrays: dr [P] (new minus old SRP range per input pulse), win [N2]
+// (fftshift(tukey) / N), kern [K] complex128 interleaved, starts [P2] (first
+// input pulse of each output pulse).  Device arrays: ph_in [P][N], ph_out
+// [P2][N2], complex64, centred frequency order.  Work buffers and FFT plans
+// are kept per device for the last (P, N, N2) seen, so repeated calls of one
+// shape allocate nothing; the call returns after the stream has finished.
+namespace {
+struct ResampleCache {
+    size_t P = 0, N = 0, N2 = 0;
+    float2 *prof = nullptr, *spec = nullptr;
+    void *work = nullptr;
+    size_t work_size = 0;
+    rocfft_plan inv = nullptr, fwd = nullptr;
+    rocfft_execution_info info = nullptr;
+    void release() {
+        dev_free(prof);
+        dev_free(spec);
+        if (work) (void)hipFree(work);
+        work = nullptr;
+        work_size = 0;
+        if (inv) rocfft_plan_destroy(inv);
+        if (fwd) rocfft_plan_destroy(fwd);
+        if (info) rocfft_execution_info_destroy(info);
+        inv = fwd = nullptr;
+        info = nullptr;
+        P = N = N2 = 0;
+    }
+};
+std::mutex g_rs_mutex;
+ResampleCache g_rs_cache[64];  // per device index
+
+int resample_setup(ResampleCache &c, size_t P, size_t N, size_t N2) {
+    if (c.P == P && c.N == N && c.N2 == N2 && c.prof) return SARHIP_OK;
+    c.release();
+    HIP_TRY(hipMalloc((void **)&c.prof, sizeof(float2) * P * N));
+    HIP_TRY(hipMalloc((void **)&c.spec, sizeof(float2) * P * N2));
+    FFT_TRY(rocfft_plan_create(&c.inv, rocfft_placement_inplace,
+                               rocfft_transform_type_complex_inverse, rocfft_precision_single, 1, &N,
+                               P, nullptr));
+    FFT_TRY(rocfft_plan_create(&c.fwd, rocfft_placement_inplace,
+                               rocfft_transform_type_complex_forward, rocfft_precision_single, 1, &N2,
+                               P, nullptr));
+    size_t w1 = 0, w2 = 0;
+    FFT_TRY(rocfft_plan_get_work_buffer_size(c.inv, &w1));
+    FFT_TRY(rocfft_plan_get_work_buffer_size(c.fwd, &w2));
+    FFT_TRY(rocfft_execution_info_create(&c.info));
+    c.work_size = std::max(w1, w2);
+    if (c.work_size) {
+        HIP_TRY(hipMalloc(&c.work, c.work_size));
+        FFT_TRY(rocfft_execution_info_set_work_buffer(c.info, c.work, c.work_size));
+    }
+    c.P = P;
+    c.N = N;
+    c.N2 = N2;
+    return SARHIP_OK;
+}
+}  // namespace
+
+int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, double centre_freq,
+                    double sample_freq, double sign_multiplier, double c_eff, size_t N2,
+                    const double *win, const double *kern, int K, const int *starts, size_t P2,
+                    void *ph_out, void *stream) {
+    if (!ph_in || !ph_out || !dr || !win || !kern || !starts || P == 0 || N == 0 || N2 == 0 ||
+        N2 > N || K <= 0 || (size_t)K > P || P > (size_t)INT32_MAX || N > (size_t)INT32_MAX ||
+        P2 > (size_t)INT32_MAX || !(c_eff > 0.0))
+        return fail(SARHIP_EINVAL, "sarhip_resample: bad argument");
+    for (size_t i = 0; i < P2; ++i)
+        if (starts[i] < 0 || (size_t)starts[i] + (size_t)K > P)
+            return fail(SARHIP_EINVAL, "sarhip_resample: window outside the input pulses");
+    std::call_once(g_fft_once, [] { g_fft_setup_status = rocfft_setup(); });
+    if (g_fft_setup_status != rocfft_status_success) return fail(SARHIP_EFFT, "rocfft_setup failed");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) return fail(SARHIP_ENODEV, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_rs_mutex);
+    ResampleCache &c = g_rs_cache[dev];
+    int rc = resample_setup(c, P, N, N2);
+    if (rc) {
+        c.release();
+        return rc;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    struct Small {
+        double *dr = nullptr;
+        float *win = nullptr;
+        float2 *kern = nullptr;
+        int *starts = nullptr;
+        ~Small() {
+            dev_free(dr);
+            dev_free(win);
+            dev_free(kern);
+            dev_free(starts);
+        }
+    } b;
+    std::vector<float> wf(N2);
+    for (size_t j = 0; j < N2; ++j) wf[j] = (float)win[j];
+    std::vector<float2> kf(K);
+    for (int k = 0; k < K; ++k) kf[k] = make_float2((float)kern[2 * k], (float)kern[2 * k + 1]);
+    HIP_TRY(hipMalloc((void **)&b.dr, sizeof(double) * P));
+    HIP_TRY(hipMalloc((void **)&b.win, sizeof(float) * N2));
+    HIP_TRY(hipMalloc((void **)&b.kern, sizeof(float2) * K));
+    HIP_TRY(hipMalloc((void **)&b.starts, sizeof(int) * std::max<size_t>(P2, 1)));
+    HIP_TRY(hipMemcpyAsync(b.dr, dr, sizeof(double) * P, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b.win, wf.data(), sizeof(float) * N2, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(b.kern, kf.data(), sizeof(float2) * K, hipMemcpyHostToDevice, s));
+    if (P2) HIP_TRY(hipMemcpyAsync(b.starts, starts, sizeof(int) * P2, hipMemcpyHostToDevice, s));
+    FFT_TRY(rocfft_execution_info_set_stream(c.info, s));
+    HIP_TRY(launch_rs_ramp((const float2 *)ph_in, (int)P, (int)N, b.dr,
+                           sample_freq / (double)N, sign_multiplier * centre_freq, 1.0 / c_eff,
+                           c.prof, s));
+    void *bp[1] = {c.prof};
+    FFT_TRY(rocfft_execute(c.inv, bp, nullptr, c.info));
+    HIP_TRY(launch_rs_crop(c.prof, (int)P, (int)N, (int)N2, b.win, c.spec, s));
+    void *bs[1] = {c.spec};
+    FFT_TRY(rocfft_execute(c.fwd, bs, nullptr, c.info));
+    if (P2)
+        HIP_TRY(launch_rs_fir(c.spec, (int)P2, (int)N2, b.kern, K, b.starts, (float2 *)ph_out, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SARHIP_OK;
+}
+
 const char *sarhip_version(void) { return "sarsolver_hip 0.1.0 (gfx950)"; }
 
 int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_times,

@@ -1,3 +1,4 @@
-from .geodesy import scene_params_enu, scene_params_classic_sar
+from .geodesy import move_srps, scene_params_enu, scene_params_classic_sar, srp_range_change
 from .k_space import OrientedBoundingBox
 from .misc import default_num_threads
+from .signal import fast_freq_downsample, get_downsample_kernel, to_db_abs

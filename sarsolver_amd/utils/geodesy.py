@@ -86,3 +86,29 @@ def scene_params_classic_sar(scene_centre_ecef: NDArray, aperture, tol: float = 
     ground = np.array([np.sqrt(1.0 - np.dot(rng, up) ** 2), np.sqrt(1.0 - np.dot(cross, up) ** 2),
                        np.dot(oop, up)]) / (2.0 * pi)
     return rot.T, obb.spans / (2.0 * pi), ground
+
+
+def srp_range_change(old_srps: NDArray, new_srp: NDArray, trans_posns: NDArray,
+                     rec_posns: NDArray) -> NDArray:
+    """Per pulse, the bistatic range to `new_srp` minus that to the pulse's old
+    SRP (utils/geodesy.py:64-66)."""
+    new_srp = np.asarray(new_srp, dtype=np.float64).reshape(1, 3)
+    new = (np.linalg.norm(trans_posns - new_srp, axis=-1)
+           + np.linalg.norm(rec_posns - new_srp, axis=-1))
+    old = (np.linalg.norm(trans_posns - old_srps, axis=-1)
+           + np.linalg.norm(rec_posns - old_srps, axis=-1))
+    return new - old
+
+
+def move_srps(phase_history: NDArray, old_srps: NDArray, new_srp: NDArray, trans_posns: NDArray,
+              rec_posns: NDArray, centre_freq: float, sample_bandwidth: float,
+              sign_factor: float = 1.0, c_eff: float = 299792458.0) -> NDArray:
+    """Rephase a centred-order phase history [P, N] to a new SRP: each sample is
+    multiplied by exp(2 pi i dr (f_bb - sign fc) / c), dr from
+    `srp_range_change`, f_bb the sample's baseband frequency
+    (utils/geodesy.py:33-73)."""
+    n = phase_history.shape[1]
+    dr = srp_range_change(old_srps, new_srp, trans_posns, rec_posns)[:, None]
+    f_bb = np.fft.fftshift(np.fft.fftfreq(n, 1.0 / sample_bandwidth))[None, :]
+    return phase_history * np.exp((2.0j * pi / c_eff) * dr * (f_bb - sign_factor * centre_freq))
+
