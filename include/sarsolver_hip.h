@@ -164,7 +164,7 @@ const char *sarhip_version(void);
  * sample).  ph_in [P][N] and ph_out [P2][N2] are device complex64 arrays in
  * centred frequency order; dr [P], win [N2] (= fftshift(tukey(N2)) / N),
  * kern [K] (complex128 interleaved) and starts [P2] are host arrays.  Work
- * buffers and FFT plans are kept per device for the last (P, N, N2) shape
+ * buffers and FFT plans are kept per device for the last (P, N, N2, P2) shape
  * (held until the process ends); the call returns once `stream` has finished.
  * Calls from several host threads are serialised. */
 int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, double centre_freq,

@@ -3,15 +3,18 @@
 // The reference (numpy/measurement.py:342-457) rebuilds each output pulse
 // from a window of K input pulses: it moves their SRP (utils/geodesy.py:33-73),
 // crops their range profiles (utils/signal.py:45-73) and sums them with a
-// complex slow-time kernel (utils/signal.py:7-25).  The first two steps act on
-// one input pulse at a time, so here they run once per input pulse instead of
-// once per window:
-//   rs_ramp_kernel : SRP phase ramp, ifftshift to FFT order         [P][N]
-//   rocFFT inverse, length N (in place)
-//   rs_crop_kernel : keep N2 profile samples, Tukey taper, 1/N       [P][N2]
-//   rocFFT forward, length N2 (in place)
-//   rs_fir_kernel  : slow-time kernel over each window, fftshift     [P2][N2]
-// All three kernels are HBM-bound elementwise passes.
+// complex slow-time kernel (utils/signal.py:7-25).  The SRP move and the
+// inverse FFT act on one input pulse at a time, so here they run once per
+// input pulse instead of once per window.  The crop, the taper, the kernel sum
+// and the forward FFT are all linear in the range samples, so the kernel sum
+// is taken on the cropped range profiles and only the P2 output pulses are
+// transformed back; the final fftshift is a phase applied before that FFT:
+//   rs_ramp_kernel   : SRP phase ramp, ifftshift to FFT order         [P][N]
+//   rocFFT inverse, length N, P rows (in place)
+//   rs_window_kernel : crop N2 samples, slow-time kernel over each
+//                      window, Tukey taper / N, fftshift phase         [P2][N2]
+//   rocFFT forward, length N2, P2 rows (in place, on the output)
+// Both kernels are HBM-bound elementwise passes.
 #include <hip/hip_runtime.h>
 
 #include "sar_device.h"
@@ -43,40 +46,40 @@ __global__ __launch_bounds__(256) void rs_ramp_kernel(const float2 *__restrict__
     out[(size_t)p * N + j] = cmulf(v, make_float2(cs, sn));
 }
 
-// out[p][j2] = win[j2] * prof[p][src(j2)], src = j2 for j2 < N2/2 (floor), else
-// N - (N2 - j2): the first floor(N2/2) and last N2 - floor(N2/2) samples
-// (signal.py:64-68).  win holds fftshift(tukey) / N (numpy's ifft scaling).
-__global__ __launch_bounds__(256) void rs_crop_kernel(const float2 *__restrict__ prof, int N,
-                                                      int N2, const float *__restrict__ win,
-                                                      float2 *__restrict__ out) {
-    const int j2 = blockIdx.x * 256 + threadIdx.x;
-    const int p = blockIdx.y;
-    if (j2 >= N2) return;
-    const int src = j2 < N2 / 2 ? j2 : N - (N2 - j2);
-    const float2 v = prof[(size_t)p * N + src];
-    const float w = win[j2];
-    out[(size_t)p * N2 + j2] = make_float2(v.x * w, v.y * w);
-}
-
-// out[i][m] = sum_k kern[k] * spec[starts[i] + k][(m - N2/2) mod N2]: the
-// slow-time kernel over window i, written in centred order (fftshift).
-__global__ __launch_bounds__(256) void rs_fir_kernel(const float2 *__restrict__ spec, int N2,
-                                                     const float2 *__restrict__ kern, int K,
-                                                     const int *__restrict__ starts,
-                                                     float2 *__restrict__ out) {
-    const int m = blockIdx.x * 256 + threadIdx.x;
+// out[i][j] = e^(2 pi i j h / N2) win[j] sum_k kern[k] prof[starts[i] + k][src(j)],
+// h = floor(N2 / 2), src(j) = j for j < h, else N - (N2 - j): the first h and
+// last N2 - h range samples of each profile (signal.py:64-68).  win holds
+// fftshift(tukey) / N (numpy's ifft scaling).  The phase e^(2 pi i j h / N2)
+// turns the forward FFT's output into centred order: fftshift(FFT(x))[m] =
+// FFT(x)[(m - h) mod N2] = FFT(x e^(2 pi i j h / N2))[m].
+__global__ __launch_bounds__(256) void rs_window_kernel(const float2 *__restrict__ prof, int N,
+                                                        int N2, const float *__restrict__ win,
+                                                        const float2 *__restrict__ kern, int K,
+                                                        const int *__restrict__ starts,
+                                                        float2 *__restrict__ out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
     const int i = blockIdx.y;
-    if (m >= N2) return;
-    const int col = (m - N2 / 2 + N2) % N2;
-    const float2 *base = spec + (size_t)starts[i] * N2 + col;
+    if (j >= N2) return;
+    const int h = N2 / 2;
+    const int src = j < h ? j : N - (N2 - j);
+    const float2 *base = prof + (size_t)starts[i] * N + src;
     float2 acc = make_float2(0.f, 0.f);
     for (int k = 0; k < K; ++k) {
-        const float2 v = base[(size_t)k * N2];
+        const float2 v = base[(size_t)k * N];
         const float2 w = kern[k];
         acc.x = fmaf(v.x, w.x, fmaf(-v.y, w.y, acc.x));
         acc.y = fmaf(v.x, w.y, fmaf(v.y, w.x, acc.y));
     }
-    out[(size_t)i * N2 + m] = acc;
+    // j h mod N2 in integers keeps the phase exact; 0 and N2/2 are +1 and -1.
+    const long long r = ((long long)j * h) % N2;
+    float2 ph = make_float2(1.f, 0.f);
+    if (2 * r == N2) {
+        ph = make_float2(-1.f, 0.f);
+    } else if (r != 0) {
+        phasor_rev((double)r / (double)N2, &ph.y, &ph.x);
+    }
+    const float wj = win[j];
+    out[(size_t)i * N2 + j] = cmulf(make_float2(acc.x * wj, acc.y * wj), ph);
 }
 
 }  // namespace
@@ -94,22 +97,12 @@ hipError_t launch_rs_ramp(const float2 *in, int P, int N, const double *dr, doub
     return hipGetLastError();
 }
 
-hipError_t launch_rs_crop(const float2 *prof, int P, int N, int N2, const float *win, float2 *out,
-                          hipStream_t s) {
-    for (int p0 = 0; p0 < P; p0 += kMaxRows) {
-        const int rows = P - p0 < kMaxRows ? P - p0 : kMaxRows;
-        rs_crop_kernel<<<dim3((N2 + 255) / 256, rows), 256, 0, s>>>(
-            prof + (size_t)p0 * N, N, N2, win, out + (size_t)p0 * N2);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_rs_fir(const float2 *spec, int P2, int N2, const float2 *kern, int K,
-                         const int *starts, float2 *out, hipStream_t s) {
+hipError_t launch_rs_window(const float2 *prof, int N, int N2, const float *win, const float2 *kern,
+                            int K, const int *starts, int P2, float2 *out, hipStream_t s) {
     for (int i0 = 0; i0 < P2; i0 += kMaxRows) {
         const int rows = P2 - i0 < kMaxRows ? P2 - i0 : kMaxRows;
-        rs_fir_kernel<<<dim3((N2 + 255) / 256, rows), 256, 0, s>>>(
-            spec, N2, kern, K, starts + i0, out + (size_t)i0 * N2);
+        rs_window_kernel<<<dim3((N2 + 255) / 256, rows), 256, 0, s>>>(
+            prof, N, N2, win, kern, K, starts + i0, out + (size_t)i0 * N2);
     }
     return hipGetLastError();
 }

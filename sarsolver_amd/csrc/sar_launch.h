@@ -68,10 +68,8 @@ hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)
 hipError_t launch_rs_ramp(const float2 *in, int P, int N, const double *dr, double fs_over_n,
                           double sgn_fc, double inv_c, float2 *out, hipStream_t s);
-hipError_t launch_rs_crop(const float2 *prof, int P, int N, int N2, const float *win, float2 *out,
-                          hipStream_t s);
-hipError_t launch_rs_fir(const float2 *spec, int P2, int N2, const float2 *kern, int K,
-                         const int *starts, float2 *out, hipStream_t s);
+hipError_t launch_rs_window(const float2 *prof, int N, int N2, const float *win, const float2 *kern,
+                            int K, const int *starts, int P2, float2 *out, hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

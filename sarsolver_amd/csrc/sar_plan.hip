@@ -372,19 +372,18 @@ const char *sarhip_last_error(void) { return g_last_error.c_str(); }
 // (fftshift(tukey) / N), kern [K] complex128 interleaved, starts [P2] (first
 // input pulse of each output pulse).  Device arrays: ph_in [P][N], ph_out
 // [P2][N2], complex64, centred frequency order.  Work buffers and FFT plans
-// are kept per device for the last (P, N, N2) seen, so repeated calls of one
+// are kept per device for the last (P, N, N2, P2) seen, so repeated calls of one
 // shape allocate nothing; the call returns after the stream has finished.
 namespace {
 struct ResampleCache {
-    size_t P = 0, N = 0, N2 = 0;
-    float2 *prof = nullptr, *spec = nullptr;
+    size_t P = 0, N = 0, N2 = 0, P2 = 0;
+    float2 *prof = nullptr;
     void *work = nullptr;
     size_t work_size = 0;
     rocfft_plan inv = nullptr, fwd = nullptr;
     rocfft_execution_info info = nullptr;
     void release() {
         dev_free(prof);
-        dev_free(spec);
         if (work) (void)hipFree(work);
         work = nullptr;
         work_size = 0;
@@ -393,26 +392,30 @@ struct ResampleCache {
         if (info) rocfft_execution_info_destroy(info);
         inv = fwd = nullptr;
         info = nullptr;
-        P = N = N2 = 0;
+        P = N = N2 = P2 = 0;
     }
 };
 std::mutex g_rs_mutex;
 ResampleCache g_rs_cache[64];  // per device index
 
-int resample_setup(ResampleCache &c, size_t P, size_t N, size_t N2) {
-    if (c.P == P && c.N == N && c.N2 == N2 && c.prof) return SARHIP_OK;
+// Inverse FFT of length N over the P input rows (in place on the profile
+// buffer); forward FFT of length N2 over the P2 output rows (in place on the
+// caller's output).
+int resample_setup(ResampleCache &c, size_t P, size_t N, size_t N2, size_t P2) {
+    if (c.P == P && c.N == N && c.N2 == N2 && c.P2 == P2 && c.prof) return SARHIP_OK;
     c.release();
     HIP_TRY(hipMalloc((void **)&c.prof, sizeof(float2) * P * N));
-    HIP_TRY(hipMalloc((void **)&c.spec, sizeof(float2) * P * N2));
     FFT_TRY(rocfft_plan_create(&c.inv, rocfft_placement_inplace,
                                rocfft_transform_type_complex_inverse, rocfft_precision_single, 1, &N,
                                P, nullptr));
-    FFT_TRY(rocfft_plan_create(&c.fwd, rocfft_placement_inplace,
-                               rocfft_transform_type_complex_forward, rocfft_precision_single, 1, &N2,
-                               P, nullptr));
     size_t w1 = 0, w2 = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(c.inv, &w1));
-    FFT_TRY(rocfft_plan_get_work_buffer_size(c.fwd, &w2));
+    if (P2) {
+        FFT_TRY(rocfft_plan_create(&c.fwd, rocfft_placement_inplace,
+                                   rocfft_transform_type_complex_forward, rocfft_precision_single,
+                                   1, &N2, P2, nullptr));
+        FFT_TRY(rocfft_plan_get_work_buffer_size(c.fwd, &w2));
+    }
     FFT_TRY(rocfft_execution_info_create(&c.info));
     c.work_size = std::max(w1, w2);
     if (c.work_size) {
@@ -422,6 +425,7 @@ int resample_setup(ResampleCache &c, size_t P, size_t N, size_t N2) {
     c.P = P;
     c.N = N;
     c.N2 = N2;
+    c.P2 = P2;
     return SARHIP_OK;
 }
 }  // namespace
@@ -444,7 +448,7 @@ int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, dou
     if (dev < 0 || dev >= 64) return fail(SARHIP_ENODEV, "device index out of range");
     std::lock_guard<std::mutex> lock(g_rs_mutex);
     ResampleCache &c = g_rs_cache[dev];
-    int rc = resample_setup(c, P, N, N2);
+    int rc = resample_setup(c, P, N, N2, P2);
     if (rc) {
         c.release();
         return rc;
@@ -480,11 +484,12 @@ int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, dou
                            c.prof, s));
     void *bp[1] = {c.prof};
     FFT_TRY(rocfft_execute(c.inv, bp, nullptr, c.info));
-    HIP_TRY(launch_rs_crop(c.prof, (int)P, (int)N, (int)N2, b.win, c.spec, s));
-    void *bs[1] = {c.spec};
-    FFT_TRY(rocfft_execute(c.fwd, bs, nullptr, c.info));
-    if (P2)
-        HIP_TRY(launch_rs_fir(c.spec, (int)P2, (int)N2, b.kern, K, b.starts, (float2 *)ph_out, s));
+    if (P2) {
+        HIP_TRY(launch_rs_window(c.prof, (int)N, (int)N2, b.win, b.kern, K, b.starts, (int)P2,
+                                 (float2 *)ph_out, s));
+        void *bo[1] = {ph_out};
+        FFT_TRY(rocfft_execute(c.fwd, bo, nullptr, c.info));
+    }
     HIP_TRY(hipStreamSynchronize(s));
     return SARHIP_OK;
 }
