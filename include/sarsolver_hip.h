@@ -105,6 +105,51 @@ int sarhip_plan_create(sarhip_plan_t *plan, size_t num_fast_times, size_t num_sl
                        double sample_frequency, double c_eff, double upsample_ratio,
                        double sign_multiplier, int interp, int precision, int device);
 
+/* As sarhip_plan_create, with the scatterer positions ([M][3] FP64) already
+ * in DEVICE memory on `device` (read on the default stream; the caller's
+ * writes must be complete).  The plan keeps its own copy in tile order.  It
+ * replaces the upload of GridSarScene.positions / SimpleSarScene positions
+ * (numpy/hypothesis.py:191-206) that the reference performs on every call. */
+int sarhip_plan_create_device_positions(sarhip_plan_t *plan, size_t num_fast_times,
+                                        size_t num_slow_times, size_t num_scatterers,
+                                        const double *transmit_posns, const double *receive_posns,
+                                        const double *stab_ref_posns, const void *d_scat_posns,
+                                        const double *waveform_fft,
+                                        const double *slow_time_weighting, double centre_frequency,
+                                        double sample_frequency, double c_eff,
+                                        double upsample_ratio, double sign_multiplier, int interp,
+                                        int precision, int device);
+
+/* As sarhip_plan_create, for the pixels of a GridSarScene generated on the
+ * device (numpy/hypothesis.py:191-206): grid_shape and grid_dimensions in
+ * (z, y, x) order, rotation row-major [3][3], centre [3]; M = product of the
+ * shape.  Positions are bit-identical to numpy's linspace/meshgrid and to
+ * R @ local + centre evaluated as an FMA chain in k order (how numpy's
+ * OpenBLAS matmul rounds it; for rotations of 0 and +-1 entries any order
+ * agrees). */
+int sarhip_plan_create_grid(sarhip_plan_t *plan, size_t num_fast_times, size_t num_slow_times,
+                            const size_t grid_shape[3], const double grid_dimensions[3],
+                            const double grid_centre[3], const double rotation[9],
+                            const double *transmit_posns, const double *receive_posns,
+                            const double *stab_ref_posns, const double *waveform_fft,
+                            const double *slow_time_weighting, double centre_frequency,
+                            double sample_frequency, double c_eff, double upsample_ratio,
+                            double sign_multiplier, int interp, int precision, int device);
+
+/* GridSarScene.positions (numpy/hypothesis.py:191-206) into device memory
+ * d_pos [M][3] FP64 (caller order), stream-ordered on `stream`. */
+int sarhip_grid_positions(const size_t grid_shape[3], const double grid_dimensions[3],
+                          const double grid_centre[3], const double rotation[9], void *d_pos,
+                          void *stream);
+
+/* generate_simple_basebanding_values (utils/k_space.py:109-134) on the device:
+ * d_out[m] = exp(-i sign ((d_pos[m] - spot) . displacement)), complex128
+ * interleaved [M]; d_pos [M][3] FP64 device memory; `displacement` is the
+ * k-space box displacement the reference computes on the host. */
+int sarhip_baseband_values(const void *d_pos, size_t num_scatterers, const double spot[3],
+                           const double displacement[3], double sign_multiplier, void *d_out,
+                           void *stream);
+
 int sarhip_plan_destroy(sarhip_plan_t plan);
 
 /* Forward projection on device buffers: amps [M] complex64 ->

@@ -50,7 +50,8 @@ EXPORTED_SYMBOLS = (
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
     "sarhip_plan_set_timing",
     "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
-    "sarhip_resample",
+    "sarhip_resample", "sarhip_plan_create_device_positions", "sarhip_plan_create_grid",
+    "sarhip_grid_positions", "sarhip_baseband_values",
     "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
     "sarhip_conv_last_error",
 )
@@ -108,6 +109,19 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_resample.argtypes = [vp, sz, sz, dp, dbl, dbl, dbl, dbl, sz, dp, dp,
                                       ctypes.c_int, ip, sz, vp, vp]
         L.sarhip_resample.restype = ctypes.c_int
+        L.sarhip_plan_create_device_positions.argtypes = [
+            ctypes.POINTER(vp), sz, sz, sz, dp, dp, dp, vp, dp, dp, dbl, dbl, dbl, dbl, dbl,
+            ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.sarhip_plan_create_device_positions.restype = ctypes.c_int
+        szp = ctypes.POINTER(ctypes.c_size_t)
+        L.sarhip_plan_create_grid.argtypes = [
+            ctypes.POINTER(vp), sz, sz, szp, dp, dp, dp, dp, dp, dp, dp, dp, dbl, dbl, dbl, dbl,
+            dbl, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.sarhip_plan_create_grid.restype = ctypes.c_int
+        L.sarhip_grid_positions.argtypes = [szp, dp, dp, dp, vp, vp]
+        L.sarhip_grid_positions.restype = ctypes.c_int
+        L.sarhip_baseband_values.argtypes = [vp, sz, dp, dp, dbl, vp, vp]
+        L.sarhip_baseband_values.restype = ctypes.c_int
         _lib = L
         return L
 
@@ -145,6 +159,101 @@ def _stream_handle(device) -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+class _GridArgs:
+    """ctypes arrays describing a GridSarScene for the device generator."""
+
+    def __init__(self, shape, dims, centre, rot):
+        self.shape = (ctypes.c_size_t * 3)(*shape)
+        self.dims = np.ascontiguousarray(dims, dtype=np.float64)
+        self.centre = np.ascontiguousarray(centre, dtype=np.float64)
+        self.rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(9)
+        self.num_points = int(shape[0]) * int(shape[1]) * int(shape[2])
+        self.key = (tuple(int(v) for v in shape), self.dims.tobytes(), self.centre.tobytes(),
+                    self.rot.tobytes())
+
+
+def grid_args(scene) -> Optional[_GridArgs]:
+    """The grid description of a GridSarScene-like object (grid_shape,
+    grid_dimensions in (z, y, x) order, grid_centre, rotation_matrix) when
+    the device generator reproduces its ``positions`` bit for bit, else None.
+
+    numpy evaluates ``-0.5 * L`` in L's own type, so edge lengths must be
+    Python numbers or FP64 (a float32 edge length gives a float32 linspace);
+    centre and rotation only need to convert to FP64 exactly, which every
+    real numpy dtype does."""
+    need = ("grid_shape", "grid_dimensions", "grid_centre", "rotation_matrix")
+    if not all(hasattr(scene, a) for a in need):
+        return None
+    shape = [int(v) for v in scene.grid_shape]
+    dims = list(scene.grid_dimensions)
+    if len(shape) != 3 or len(dims) != 3 or min(shape) < 1:
+        return None
+    for d in dims:
+        if not (isinstance(d, (int, float, np.integer)) or
+                (isinstance(d, np.floating) and np.asarray(d).dtype == np.float64)):
+            return None
+        if isinstance(d, (int, np.integer)) and abs(int(d)) > 2 ** 53:
+            return None
+    centre = np.asarray(scene.grid_centre)
+    rot = np.asarray(scene.rotation_matrix)
+    if centre.shape != (3,) or rot.shape != (3, 3):
+        return None
+    if centre.dtype.kind not in "fiu" or rot.dtype.kind not in "fiu":
+        return None
+    return _GridArgs(shape, [float(d) for d in dims], centre, rot)
+
+
+class _GridScene:
+    """Minimal GridSarScene stand-in rebuilt from stored grid arguments."""
+
+    def __init__(self, g: _GridArgs):
+        self.grid_shape = tuple(g.shape)
+        self.grid_dimensions = [float(v) for v in g.dims]
+        self.grid_centre = g.centre
+        self.rotation_matrix = g.rot.reshape(3, 3)
+
+
+def src_scene_of(g: _GridArgs) -> _GridScene:
+    return _GridScene(g)
+
+
+def grid_positions(scene, device=None):
+    """GridSarScene positions generated on the device: [M, 3] float64 torch
+    tensor, bit-identical to ``scene.positions``."""
+    torch = _torch()
+    g = grid_args(scene)
+    if g is None:
+        raise ValueError("scene is not a grid the device generator reproduces exactly")
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    out = torch.empty((g.num_points, 3), dtype=torch.float64, device=dev)
+    with torch.cuda.device(dev):
+        rc = load_library().sarhip_grid_positions(g.shape, _dptr(g.dims), _dptr(g.centre),
+                                                   _dptr(g.rot), ctypes.c_void_p(out.data_ptr()),
+                                                   _stream_handle(dev))
+    check(rc, "sarhip_grid_positions")
+    return out
+
+
+def baseband_values(positions, spot, displacement, sign_multiplier: float):
+    """exp(-i sign ((positions - spot) . displacement)) on the device:
+    positions [M, 3] float64 CUDA tensor -> [M] complex128 tensor."""
+    torch = _torch()
+    if not (isinstance(positions, torch.Tensor) and positions.is_cuda and
+            positions.dtype == torch.float64 and positions.dim() == 2 and positions.shape[1] == 3):
+        raise TypeError("positions must be an [M, 3] float64 tensor on a GPU")
+    pos = positions.contiguous()
+    sp = _f64(spot).reshape(3)
+    dp_ = _f64(displacement).reshape(3)
+    out = torch.empty((pos.shape[0],), dtype=torch.complex128, device=pos.device)
+    with torch.cuda.device(pos.device):
+        rc = load_library().sarhip_baseband_values(ctypes.c_void_p(pos.data_ptr()), pos.shape[0],
+                                                    _dptr(sp), _dptr(dp_), float(sign_multiplier),
+                                                    ctypes.c_void_p(out.data_ptr()),
+                                                    _stream_handle(pos.device))
+    check(rc, "sarhip_baseband_values")
+    return out
+
+
 class SarPlan:
     """A device plan: geometry uploaded once, operators run on device tensors.
 
@@ -158,6 +267,9 @@ class SarPlan:
                  slow_time_weighting, centre_frequency: float, sample_frequency: float,
                  c_eff: float, upsample_ratio: float = 128.0, sign_multiplier: float = 1.0,
                  interp: str = "nearest_ref", precision: str = "fp64", device=None):
+        """``scat_posns`` is an [M, 3] host array, an [M, 3] float64 CUDA tensor
+        on the plan's device (read once; the plan keeps its own copy), or a
+        GridSarScene, whose positions are then generated on the device."""
         torch = _torch()
         L = load_library()
         if interp not in INTERP:
@@ -167,32 +279,64 @@ class SarPlan:
         tx = _f64(transmit_posns).reshape(-1)
         rx = _f64(receive_posns).reshape(-1)
         srp = _f64(stab_ref_posns).reshape(-1)
-        pos = _f64(scat_posns).reshape(-1)
         W = _c128_interleaved(waveform_fft)
         w = _c128_interleaved(slow_time_weighting)
-        self.geometry = {"tx": tx.reshape(-1, 3), "rx": rx.reshape(-1, 3),
-                         "srp": srp.reshape(-1, 3), "pos": pos.reshape(-1, 3)}
-        self.num_slow_times = tx.shape[0] // 3
-        self.num_fast_times = W.shape[0] // 2
-        self.num_scatterers = pos.shape[0] // 3
-        P, N, M = self.num_slow_times, self.num_fast_times, self.num_scatterers
-        if tx.shape[0] != 3 * P or rx.shape[0] != 3 * P or srp.shape[0] != 3 * P:
-            raise ValueError("transmitter, receiver and SRP arrays must all be [P, 3]")
-        if pos.shape[0] != 3 * M or M == 0:
-            raise ValueError("scatterer positions must be a non-empty [M, 3] array")
-        if w.shape[0] != 2 * P:
-            raise ValueError("slow_time_weighting must have one entry per pulse")
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = torch.device(device)
+        grid = grid_args(scat_posns)
+        if grid is not None:
+            kind, M = "grid", grid.num_points
+            self._pos_source = grid
+        elif isinstance(scat_posns, torch.Tensor):
+            t = scat_posns
+            if not t.is_cuda or t.device != self.device:
+                raise ValueError(f"device positions are on {t.device}, plan is on {self.device}")
+            if t.dtype != torch.float64 or t.dim() != 2 or t.shape[1] != 3:
+                raise ValueError("device positions must be an [M, 3] float64 tensor")
+            kind, M = "device", int(t.shape[0])
+            t = t.contiguous()
+            self._pos_source = t
+        else:
+            pos = _f64(scat_posns).reshape(-1)
+            if pos.shape[0] % 3:
+                raise ValueError("scatterer positions must be a non-empty [M, 3] array")
+            kind, M = "host", pos.shape[0] // 3
+            self._pos_source = pos.reshape(-1, 3)
+        self.position_source = kind
+        self._geometry = {"tx": tx.reshape(-1, 3), "rx": rx.reshape(-1, 3),
+                          "srp": srp.reshape(-1, 3)}
+        self.num_slow_times = tx.shape[0] // 3
+        self.num_fast_times = W.shape[0] // 2
+        self.num_scatterers = M
+        P, N = self.num_slow_times, self.num_fast_times
+        if tx.shape[0] != 3 * P or rx.shape[0] != 3 * P or srp.shape[0] != 3 * P:
+            raise ValueError("transmitter, receiver and SRP arrays must all be [P, 3]")
+        if M == 0:
+            raise ValueError("scatterer positions must be a non-empty [M, 3] array")
+        if w.shape[0] != 2 * P:
+            raise ValueError("slow_time_weighting must have one entry per pulse")
         self.interp, self.precision = interp, precision
         handle = ctypes.c_void_p()
-        rc = L.sarhip_plan_create(ctypes.byref(handle), N, P, M, _dptr(tx), _dptr(rx), _dptr(srp),
-                                  _dptr(pos), _dptr(W), _dptr(w), float(centre_frequency),
-                                  float(sample_frequency), float(c_eff), float(upsample_ratio),
-                                  float(sign_multiplier), INTERP[interp], PRECISION[precision],
-                                  self.device.index or 0)
-        check(rc, "sarhip_plan_create")
+        common = (_dptr(W), _dptr(w), float(centre_frequency), float(sample_frequency),
+                  float(c_eff), float(upsample_ratio), float(sign_multiplier), INTERP[interp],
+                  PRECISION[precision], self.device.index or 0)
+        if kind == "grid":
+            rc = L.sarhip_plan_create_grid(ctypes.byref(handle), N, P, grid.shape,
+                                           _dptr(grid.dims), _dptr(grid.centre), _dptr(grid.rot),
+                                           _dptr(tx), _dptr(rx), _dptr(srp), *common)
+            check(rc, "sarhip_plan_create_grid")
+        elif kind == "device":
+            # the library reads the positions on the default stream
+            torch.cuda.current_stream(self.device).synchronize()
+            rc = L.sarhip_plan_create_device_positions(
+                ctypes.byref(handle), N, P, M, _dptr(tx), _dptr(rx), _dptr(srp),
+                ctypes.c_void_p(t.data_ptr()), *common)
+            check(rc, "sarhip_plan_create_device_positions")
+        else:
+            rc = L.sarhip_plan_create(ctypes.byref(handle), N, P, M, _dptr(tx), _dptr(rx),
+                                      _dptr(srp), _dptr(pos), *common)
+            check(rc, "sarhip_plan_create")
         self._handle = handle
         nw = ctypes.c_size_t()
         by_dx = ctypes.c_double()
@@ -213,6 +357,21 @@ class SarPlan:
                        "fwd_pulses_per_group": vals[4].value, "series_err_bins": err.value,
                        "fwd_region_tiles": reg.value}
         self.effective_precision = self.layout["precision"]
+
+    @property
+    def geometry(self):
+        """Host copies of the geometry: tx, rx, srp [P, 3] and pos [M, 3]
+        (pos is brought to the host on first use for device and grid plans)."""
+        if "pos" not in self._geometry:
+            src = self._pos_source
+            if self.position_source == "host":
+                pos = src
+            elif self.position_source == "device":
+                pos = src.cpu().numpy()
+            else:
+                pos = grid_positions(src_scene_of(src), self.device).cpu().numpy()
+            self._geometry["pos"] = pos
+        return self._geometry
 
     def close(self) -> None:
         if getattr(self, "_handle", None):

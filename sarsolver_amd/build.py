@@ -16,7 +16,8 @@ CSRC = PKG / "csrc"
 LIB_NAME = "libsarsolver_hip.so"
 LIB_PATH = PKG / LIB_NAME
 
-SOURCES = ["sar_kernels.hip", "sar_plan.hip", "conv_kernels.hip", "resample_kernels.hip"]
+SOURCES = ["sar_kernels.hip", "sar_plan.hip", "conv_kernels.hip", "resample_kernels.hip",
+           "scene_kernels.hip"]
 
 # -ffp-contract=off: FP64 range arithmetic keeps the reference's roundings.
 # Packed FP32 instructions are disabled for all device code.

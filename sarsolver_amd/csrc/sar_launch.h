@@ -53,6 +53,33 @@ struct KernelParams {
     bool planar;  // every pixel's offset from its tile centre has dz == 0
 };
 
+// A regular grid filling a rotated, translated cuboid (GridSarScene):
+// sizes and edge lengths in (z, y, x) order, rotation row-major.
+struct GridSpec {
+    int nz, ny, nx;
+    double lz, ly, lx;
+    double rot[9];
+    double centre[3];
+};
+
+// scene set-up (scene_kernels.hip)
+hipError_t launch_grid_positions(const GridSpec &g, double *pos, hipStream_t s);
+hipError_t launch_bbox(const double *pos, int M, double *part, int part_blocks, double *out,
+                       hipStream_t s);
+hipError_t launch_morton(const double *pos, int M, const double lo[3], double qscale,
+                         unsigned long long *keys, int *idx, hipStream_t s);
+hipError_t sort_pairs(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
+                      unsigned long long *keys_out, const int *vals_in, int *vals_out, int M,
+                      hipStream_t s);
+hipError_t launch_gather_sorted(const double *pos, const int *perm, int M, double *sx, double *sy,
+                                double *sz, hipStream_t s);
+hipError_t launch_tile_stats(const double *sx, const double *sy, const double *sz, int M, int len,
+                             bool snap, double4 *centre_r, double4 *half_planar, hipStream_t s);
+hipError_t launch_pixrec(const double *sx, const double *sy, const double *sz, int M,
+                         const double4 *tiles, PixRec *recs, hipStream_t s);
+hipError_t launch_baseband(const double *pos, int M, const double spot[3], const double disp[3],
+                           double sgn, double2 *out, hipStream_t s);
+
 hipError_t launch_window_pad(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N,
                              int nw, int ld, int P, int centred, float2 *K, hipStream_t s);
 hipError_t launch_wrap_pad(float2 *prof, int nw, int ld, int P, hipStream_t s);

@@ -62,17 +62,6 @@ double host_distance(const double *a, const double *x) {
     return std::sqrt(dsq);
 }
 
-// Interleave the low 21 bits of three integers (3-D Morton code).
-uint64_t spread21(uint64_t v) {
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
-}
-
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
@@ -90,6 +79,24 @@ void dev_free(T *&p) {
     if (p) (void)hipFree((void *)p);
     p = nullptr;
 }
+
+// Device buffer freed when it goes out of scope (plan set-up temporaries).
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    hipError_t alloc(size_t n) { return hipMalloc((void **)&p, sizeof(T) * n); }
+    ~DevBuf() {
+        if (p) (void)hipFree((void *)p);
+    }
+};
+
+// Where a plan's pixel positions come from: a host [M][3] array, a device
+// [M][3] array, or a grid generated on the device.
+struct PosSource {
+    enum Kind { HOST, DEVICE, GRID } kind = HOST;
+    const double *ptr = nullptr;
+    GridSpec grid{};
+};
 
 constexpr size_t kLdsBudget = 64 * 1024;
 
@@ -247,55 +254,6 @@ int begin_timing(sarhip_plan_t pl, int which) {
 int record(sarhip_plan_t pl, int i, hipStream_t s) {
     if (pl->ev) HIP_TRY(hipEventRecord(pl->ev[i], s));
     return SARHIP_OK;
-}
-
-// Tiles: consecutive runs of `len` pixels in Morton order.  Centre = bounding-
-// box midpoint, rounded to a multiple of 2^-10 m when `snap` (so that pixel
-// offsets from it are exact in FP64); radius = max distance to the centre,
-// inflated slightly.  Tiles without a finite pixel get centre 0, radius 0.
-void make_tiles(const std::vector<double> &sx, const std::vector<double> &sy,
-                const std::vector<double> &sz, size_t M, size_t len, bool snap,
-                std::vector<double4> &tiles, double &rad_max,
-                std::vector<double4> *half = nullptr) {
-    const size_t nt = (M + len - 1) / len;
-    tiles.assign(nt, double4{0, 0, 0, 0});
-    if (half) half->assign(nt, double4{0, 0, 0, 0});
-    rad_max = 0.0;
-    for (size_t t = 0; t < nt; ++t) {
-        const size_t a = t * len, b = std::min(M, a + len);
-        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-        bool any = false;
-        for (size_t i = a; i < b; ++i) {
-            const double v[3] = {sx[i], sy[i], sz[i]};
-            if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]))) continue;
-            any = true;
-            for (int k = 0; k < 3; ++k) {
-                lo[k] = std::min(lo[k], v[k]);
-                hi[k] = std::max(hi[k], v[k]);
-            }
-        }
-        if (!any) continue;
-        double c[3];
-        for (int k = 0; k < 3; ++k) {
-            c[k] = 0.5 * (lo[k] + hi[k]);
-            if (snap) c[k] = std::ldexp(std::nearbyint(std::ldexp(c[k], 10)), -10);
-        }
-        double r = 0.0;
-        for (size_t i = a; i < b; ++i) {
-            const double dx = sx[i] - c[0], dy = sy[i] - c[1], dz = sz[i] - c[2];
-            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
-            if (std::isfinite(d)) r = std::max(r, d);
-        }
-        r = r * (1.0 + 1e-9) + 1e-9;
-        tiles[t] = double4{c[0], c[1], c[2], r};
-        rad_max = std::max(rad_max, r);
-        if (half) {
-            double h[3];
-            for (int k = 0; k < 3; ++k)
-                h[k] = std::max(hi[k] - c[k], c[k] - lo[k]) * (1.0 + 1e-9) + 1e-9;
-            (*half)[t] = double4{h[0], h[1], h[2], 0.0};
-        }
-    }
 }
 
 // Forward LDS window length (bins) for regions with bounding boxes `reg` /
@@ -496,12 +454,13 @@ int sarhip_resample(const void *ph_in, size_t P, size_t N, const double *dr, dou
 
 const char *sarhip_version(void) { return "sarsolver_hip 0.1.0 (gfx950)"; }
 
-int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_times,
-                       size_t num_scatterers, const double *tx, const double *rx,
-                       const double *srp, const double *pos, const double *waveform_fft,
-                       const double *slow_time_weighting, double centre_frequency,
-                       double sample_frequency, double c_eff, double upsample_ratio,
-                       double sign_multiplier, int interp, int precision, int device) {
+namespace {
+int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_times,
+                     size_t num_scatterers, const double *tx, const double *rx, const double *srp,
+                     const PosSource &src, const double *waveform_fft,
+                     const double *slow_time_weighting, double centre_frequency,
+                     double sample_frequency, double c_eff, double upsample_ratio,
+                     double sign_multiplier, int interp, int precision, int device) {
     if (!out) return fail(SARHIP_EINVAL, "plan output pointer is NULL");
     *out = nullptr;
     if (interp < 0 || interp > 2) return fail(SARHIP_EINVAL, "interp must be 0, 1 or 2");
@@ -509,7 +468,8 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         return fail(SARHIP_EINVAL, "precision must be 0 (fp64), 1 (series), 2 (hilo) or 3 (fp32)");
     const size_t N = num_fast_times, P = num_slow_times, M = num_scatterers;
     if (N == 0 || P == 0 || M == 0) return fail(SARHIP_EINVAL, "N, P and M must be positive");
-    if (!tx || !rx || !srp || !pos || !waveform_fft || !slow_time_weighting)
+    if (!tx || !rx || !srp || (src.kind != PosSource::GRID && !src.ptr) || !waveform_fft ||
+        !slow_time_weighting)
         return fail(SARHIP_EINVAL, "NULL geometry pointer");
     if (!(upsample_ratio > 0.0) || !std::isfinite(upsample_ratio))
         return fail(SARHIP_EINVAL, "upsample_ratio must be positive and finite");
@@ -585,50 +545,91 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
     pl->Pv = (int)rows.size();
     if (rows.empty()) rows.push_back(0);  // keeps device arrays non-empty; Pv stays 0
 
-    // Tile (Morton) order of the pixels.
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t m = 0; m < M; ++m) {
-        const double *v = pos + 3 * m;
-        if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]))) continue;
+    // Pixel positions on the device, [M][3] FP64: uploaded from the host, the
+    // caller's device array, or generated from a grid description.
+    DevBuf<double> pos_buf;
+    const double *d_pos = nullptr;
+    if (src.kind == PosSource::HOST) {
+        HIP_TRY(pos_buf.alloc(3 * M));
+        HIP_TRY(hipMemcpy(pos_buf.p, src.ptr, sizeof(double) * 3 * M, hipMemcpyHostToDevice));
+        d_pos = pos_buf.p;
+    } else if (src.kind == PosSource::DEVICE) {
+        d_pos = src.ptr;
+    } else {
+        HIP_TRY(pos_buf.alloc(3 * M));
+        HIP_TRY(launch_grid_positions(src.grid, pos_buf.p, nullptr));
+        d_pos = pos_buf.p;
+    }
+
+    // Tile (Morton) order: bounding box of the finite pixels, a 21-bit
+    // quantisation per axis, and a stable radix sort of (key, index), which
+    // orders the pixels as sorting the pairs would.
+    double lo[3], hi[3];
+    {
+        DevBuf<double> part, bb;
+        HIP_TRY(part.alloc(6 * 1024));
+        HIP_TRY(bb.alloc(6));
+        HIP_TRY(launch_bbox(d_pos, (int)M, part.p, 1024, bb.p, nullptr));
+        double h[6];
+        HIP_TRY(hipMemcpy(h, bb.p, sizeof(h), hipMemcpyDeviceToHost));
         for (int k = 0; k < 3; ++k) {
-            lo[k] = std::min(lo[k], v[k]);
-            hi[k] = std::max(hi[k], v[k]);
+            lo[k] = h[k];
+            hi[k] = h[k + 3];
         }
     }
     double span = 0.0;
     for (int k = 0; k < 3; ++k)
         if (hi[k] > lo[k]) span = std::max(span, hi[k] - lo[k]);
     const double qscale = span > 0.0 ? (double)((1 << 21) - 1) / span : 0.0;
-    std::vector<std::pair<uint64_t, int>> keys(M);
-    for (size_t m = 0; m < M; ++m) {
-        const double *v = pos + 3 * m;
-        uint64_t code = ~0ULL;
-        if (std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2])) {
-            uint64_t q[3];
-            for (int k = 0; k < 3; ++k) {
-                double t = (v[k] - lo[k]) * qscale;
-                t = std::min(std::max(t, 0.0), (double)((1 << 21) - 1));
-                q[k] = (uint64_t)t;
-            }
-            code = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
-        }
-        keys[m] = {code, (int)m};
+    {
+        DevBuf<unsigned long long> keys_in, keys_out;
+        DevBuf<int> idx_in;
+        HIP_TRY(keys_in.alloc(M));
+        HIP_TRY(keys_out.alloc(M));
+        HIP_TRY(idx_in.alloc(M));
+        HIP_TRY(hipMalloc((void **)&pl->d_perm, sizeof(int) * M));
+        HIP_TRY(launch_morton(d_pos, (int)M, lo, qscale, keys_in.p, idx_in.p, nullptr));
+        size_t temp_bytes = 0;
+        HIP_TRY(sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, idx_in.p, pl->d_perm,
+                           (int)M, nullptr));
+        DevBuf<unsigned char> temp;
+        HIP_TRY(temp.alloc(std::max<size_t>(temp_bytes, 1)));
+        HIP_TRY(sort_pairs(temp.p, temp_bytes, keys_in.p, keys_out.p, idx_in.p, pl->d_perm,
+                           (int)M, nullptr));
+        HIP_TRY(hipStreamSynchronize(nullptr));  // before the key buffers are freed
     }
-    std::sort(keys.begin(), keys.end());
-    std::vector<int> perm(M);
-    std::vector<double> sx(M), sy(M), sz(M);
-    for (size_t i = 0; i < M; ++i) {
-        const int m = keys[i].second;
-        perm[i] = m;
-        sx[i] = pos[3 * (size_t)m];
-        sy[i] = pos[3 * (size_t)m + 1];
-        sz[i] = pos[3 * (size_t)m + 2];
-    }
+    HIP_TRY(hipMalloc((void **)&pl->d_px, sizeof(double) * M));
+    HIP_TRY(hipMalloc((void **)&pl->d_py, sizeof(double) * M));
+    HIP_TRY(hipMalloc((void **)&pl->d_pz, sizeof(double) * M));
+    HIP_TRY(launch_gather_sorted(d_pos, pl->d_perm, (int)M, pl->d_px, pl->d_py, pl->d_pz, nullptr));
+
+    // Tiles of `len` consecutive pixels (centre, radius; half-extents,
+    // planarity), computed on the device and copied back (O(tiles)).
+    auto tile_stats = [&](size_t len, bool snap, std::vector<double4> &c,
+                          std::vector<double4> &h) -> int {
+        const size_t nt = (M + len - 1) / len;
+        DevBuf<double4> dc, dh;
+        HIP_TRY(dc.alloc(nt));
+        HIP_TRY(dh.alloc(nt));
+        HIP_TRY(launch_tile_stats(pl->d_px, pl->d_py, pl->d_pz, (int)M, (int)len, snap, dc.p,
+                                  dh.p, nullptr));
+        c.resize(nt);
+        h.resize(nt);
+        HIP_TRY(hipMemcpy(c.data(), dc.p, sizeof(double4) * nt, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(h.data(), dh.p, sizeof(double4) * nt, hipMemcpyDeviceToHost));
+        return SARHIP_OK;
+    };
 
     // Series tiles (also the forward workgroup tiles) and per-pixel records.
-    std::vector<double4> tiles;
+    std::vector<double4> tiles, tile_half;
+    int rc;
+    if ((rc = tile_stats(kTilePixels, true, tiles, tile_half))) return rc;
     double rad_t = 0.0;
-    make_tiles(sx, sy, sz, M, kTilePixels, true, tiles, rad_t);
+    pl->planar = true;
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        rad_t = std::max(rad_t, tiles[t].w);
+        pl->planar = pl->planar && tile_half[t].w != 0.0;
+    }
     pl->num_tiles = (int)tiles.size();
     // Within a tile of radius r each leg's range moves by at most r (triangle
     // inequality), so |u - u(tile centre)| <= 2 r by_dx; series truncation and
@@ -643,22 +644,12 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         // as the contiguous one only for aligned row starts
         pl->ld = (nw + (size_t)tail + 511) / 512 * 512;
     }
-    std::vector<PixRec> recs(M);
-    for (size_t i = 0; i < M; ++i) {
-        const double4 &c = tiles[i / kTilePixels];
-        PixRec &r = recs[i];
-        const bool fin = std::isfinite(sx[i]) && std::isfinite(sy[i]) && std::isfinite(sz[i]);
-        r.x = fin ? sx[i] : c.x;
-        r.y = fin ? sy[i] : c.y;
-        r.z = fin ? sz[i] : c.z;
-        r.dx = r.x - c.x;
-        r.dy = r.y - c.y;
-        r.dz = r.z - c.z;
-        r.d2 = std::fma(r.dx, r.dx, std::fma(r.dy, r.dy, r.dz * r.dz));
-        r.ok = fin ? 1.0 : 0.0;
-    }
-    pl->planar = true;
-    for (size_t i = 0; i < M && pl->planar; ++i) pl->planar = recs[i].dz == 0.0;
+    HIP_TRY(hipMalloc((void **)&pl->d_tile_c, sizeof(double4) * tiles.size()));
+    HIP_TRY(hipMemcpy(pl->d_tile_c, tiles.data(), sizeof(double4) * tiles.size(),
+                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc((void **)&pl->d_pixrec, sizeof(PixRec) * M));
+    HIP_TRY(launch_pixrec(pl->d_px, pl->d_py, pl->d_pz, (int)M, pl->d_tile_c, pl->d_pixrec,
+                          nullptr));
 
     // Series modes need the truncation term far below the tie margin; if the
     // geometry does not allow it the series mode falls back to exact FP64.
@@ -697,8 +688,7 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
             for (int R : {1, 2, 4, 8, 16}) {
                 if (forced_r > 0 && R != forced_r) continue;
                 std::vector<double4> reg, hal;
-                double rad = 0.0;
-                make_tiles(sx, sy, sz, M, (size_t)R * kTilePixels, false, reg, rad, &hal);
+                if ((rc = tile_stats((size_t)R * kTilePixels, false, reg, hal))) return rc;
                 const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
                 if (wd > 1e6) continue;
                 const int w = (int)wd;
@@ -724,9 +714,7 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
             best_g = 32;
             best_w = 0;
         }
-        double rad = 0.0;
-        make_tiles(sx, sy, sz, M, (size_t)best_r * kTilePixels, false, regions, rad,
-                   &region_half);
+        if ((rc = tile_stats((size_t)best_r * kTilePixels, false, regions, region_half))) return rc;
         pl->region_tiles = best_r;
         pl->fwd_pulses = best_g;
         pl->fwd_groups = (std::max(pl->Pv, 1) + best_g - 1) / best_g;
@@ -753,16 +741,8 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
         HIP_TRY(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
         return SARHIP_OK;
     };
-    int rc;
     if ((rc = upload((void **)&pl->d_pc, pc.data(), sizeof(PulseConst) * P))) return rc;
     if ((rc = upload((void **)&pl->d_rows, rows.data(), sizeof(int) * rows.size()))) return rc;
-    if ((rc = upload((void **)&pl->d_px, sx.data(), sizeof(double) * M))) return rc;
-    if ((rc = upload((void **)&pl->d_py, sy.data(), sizeof(double) * M))) return rc;
-    if ((rc = upload((void **)&pl->d_pz, sz.data(), sizeof(double) * M))) return rc;
-    if ((rc = upload((void **)&pl->d_perm, perm.data(), sizeof(int) * M))) return rc;
-    if ((rc = upload((void **)&pl->d_tile_c, tiles.data(), sizeof(double4) * tiles.size())))
-        return rc;
-    if ((rc = upload((void **)&pl->d_pixrec, recs.data(), sizeof(PixRec) * M))) return rc;
     if ((rc = upload((void **)&pl->d_region_c, regions.data(), sizeof(double4) * regions.size())))
         return rc;
     if ((rc = upload((void **)&pl->d_region_h, region_half.data(),
@@ -833,6 +813,100 @@ int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slo
 
     owner.p = nullptr;
     *out = pl;
+    return SARHIP_OK;
+}
+}  // namespace
+
+int sarhip_plan_create(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_times,
+                       size_t num_scatterers, const double *tx, const double *rx,
+                       const double *srp, const double *pos, const double *waveform_fft,
+                       const double *slow_time_weighting, double centre_frequency,
+                       double sample_frequency, double c_eff, double upsample_ratio,
+                       double sign_multiplier, int interp, int precision, int device) {
+    PosSource src;
+    src.kind = PosSource::HOST;
+    src.ptr = pos;
+    return plan_create_impl(out, num_fast_times, num_slow_times, num_scatterers, tx, rx, srp, src,
+                            waveform_fft, slow_time_weighting, centre_frequency, sample_frequency,
+                            c_eff, upsample_ratio, sign_multiplier, interp, precision, device);
+}
+
+int sarhip_plan_create_device_positions(sarhip_plan_t *out, size_t num_fast_times,
+                                        size_t num_slow_times, size_t num_scatterers,
+                                        const double *tx, const double *rx, const double *srp,
+                                        const void *d_pos, const double *waveform_fft,
+                                        const double *slow_time_weighting, double centre_frequency,
+                                        double sample_frequency, double c_eff,
+                                        double upsample_ratio, double sign_multiplier, int interp,
+                                        int precision, int device) {
+    PosSource src;
+    src.kind = PosSource::DEVICE;
+    src.ptr = (const double *)d_pos;
+    return plan_create_impl(out, num_fast_times, num_slow_times, num_scatterers, tx, rx, srp, src,
+                            waveform_fft, slow_time_weighting, centre_frequency, sample_frequency,
+                            c_eff, upsample_ratio, sign_multiplier, interp, precision, device);
+}
+
+namespace {
+int grid_spec(const size_t shape[3], const double dims[3], const double centre[3],
+              const double rotation[9], GridSpec &g, size_t &M) {
+    if (!shape || !dims || !centre || !rotation) return fail(SARHIP_EINVAL, "NULL grid argument");
+    if (shape[0] == 0 || shape[1] == 0 || shape[2] == 0 || shape[0] > (size_t)INT32_MAX ||
+        shape[1] > (size_t)INT32_MAX || shape[2] > (size_t)INT32_MAX)
+        return fail(SARHIP_EINVAL, "grid shape entries must be positive");
+    const double m = (double)shape[0] * (double)shape[1] * (double)shape[2];
+    if (m > (double)INT32_MAX) return fail(SARHIP_EINVAL, "grid has more than 2^31 - 1 points");
+    g.nz = (int)shape[0];
+    g.ny = (int)shape[1];
+    g.nx = (int)shape[2];
+    g.lz = dims[0];
+    g.ly = dims[1];
+    g.lx = dims[2];
+    for (int k = 0; k < 9; ++k) g.rot[k] = rotation[k];
+    for (int k = 0; k < 3; ++k) g.centre[k] = centre[k];
+    M = shape[0] * shape[1] * shape[2];
+    return SARHIP_OK;
+}
+}  // namespace
+
+int sarhip_plan_create_grid(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_times,
+                            const size_t grid_shape[3], const double grid_dimensions[3],
+                            const double grid_centre[3], const double rotation[9],
+                            const double *tx, const double *rx, const double *srp,
+                            const double *waveform_fft, const double *slow_time_weighting,
+                            double centre_frequency, double sample_frequency, double c_eff,
+                            double upsample_ratio, double sign_multiplier, int interp,
+                            int precision, int device) {
+    if (out) *out = nullptr;
+    PosSource src;
+    src.kind = PosSource::GRID;
+    size_t M = 0;
+    int rc = grid_spec(grid_shape, grid_dimensions, grid_centre, rotation, src.grid, M);
+    if (rc) return rc;
+    return plan_create_impl(out, num_fast_times, num_slow_times, M, tx, rx, srp, src,
+                            waveform_fft, slow_time_weighting, centre_frequency, sample_frequency,
+                            c_eff, upsample_ratio, sign_multiplier, interp, precision, device);
+}
+
+int sarhip_grid_positions(const size_t grid_shape[3], const double grid_dimensions[3],
+                          const double grid_centre[3], const double rotation[9], void *d_pos,
+                          void *stream) {
+    GridSpec g;
+    size_t M = 0;
+    int rc = grid_spec(grid_shape, grid_dimensions, grid_centre, rotation, g, M);
+    if (rc) return rc;
+    if (!d_pos) return fail(SARHIP_EINVAL, "NULL output pointer");
+    HIP_TRY(launch_grid_positions(g, (double *)d_pos, (hipStream_t)stream));
+    return SARHIP_OK;
+}
+
+int sarhip_baseband_values(const void *d_pos, size_t num_scatterers, const double spot[3],
+                           const double displacement[3], double sign_multiplier, void *d_out,
+                           void *stream) {
+    if (!d_pos || !d_out || !spot || !displacement) return fail(SARHIP_EINVAL, "NULL argument");
+    if (num_scatterers > (size_t)INT32_MAX) return fail(SARHIP_EINVAL, "too many scatterers");
+    HIP_TRY(launch_baseband((const double *)d_pos, (int)num_scatterers, spot, displacement,
+                            sign_multiplier, (double2 *)d_out, (hipStream_t)stream));
     return SARHIP_OK;
 }
 

@@ -21,7 +21,7 @@ def single_adjoint_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarAper
     """Backprojected amplitudes [M] (complex128) of a phase history [P, N] in centred
     frequency order (adjoint.py:13-44)."""
     dev = device()
-    plan = plan_for(scene.positions, measurements, upsample_ratio, interp, precision, dev)
+    plan = plan_for(scene, measurements, upsample_ratio, interp, precision, dev)
     x = plan.adjoint(_ph_to_device(adjoint_amplitudes, dev), order="centred")
     return x.cpu().numpy().astype(np.complex128)
 
@@ -37,14 +37,13 @@ def multi_adjoint_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarApert
     dev = device()
     blocks = measurements.to_blocks(num_calculation_blocks)
     y_blocks = block_phase_history_array(np.asarray(adjoint_amplitudes), num_calculation_blocks)
-    positions = scene.positions
     it = list(zip(blocks, y_blocks))
     if verbose:
         from tqdm import tqdm
         it = tqdm(it)
     total = None
     for blk, yb in it:
-        plan = plan_for(positions, blk, upsample_ratio, interp, precision, dev)
+        plan = plan_for(scene, blk, upsample_ratio, interp, precision, dev)
         part = plan.adjoint(_ph_to_device(yb, dev), order="centred").cpu().numpy()
         part = part.astype(np.complex128)
         total = part if total is None else total + part

@@ -23,7 +23,7 @@ def single_forward_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarAper
     scattering amplitudes (forward.py:13-44).  Device arithmetic is complex64 with
     FP64 ranges; inputs are cast at this boundary."""
     dev = device()
-    plan = plan_for(scene.positions, measurements, upsample_ratio, interp, precision, dev)
+    plan = plan_for(scene, measurements, upsample_ratio, interp, precision, dev)
     y = plan.forward(_to_device_c64(scattering_amplitudes, dev), order="centred")
     return y.cpu().numpy().astype(np.complex128)
 
@@ -39,7 +39,6 @@ def multi_forward_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarApert
     covers every pixel)."""
     dev = device()
     blocks = measurements.to_blocks(num_calculation_blocks)
-    positions = scene.positions
     x = _to_device_c64(scattering_amplitudes, dev)
     it = blocks
     if verbose:
@@ -47,6 +46,6 @@ def multi_forward_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarApert
         it = tqdm(blocks)
     chunks = []
     for blk in it:
-        plan = plan_for(positions, blk, upsample_ratio, interp, precision, dev)
+        plan = plan_for(scene, blk, upsample_ratio, interp, precision, dev)
         chunks.append(plan.forward(x, order="centred").cpu().numpy().astype(np.complex128))
     return np.concatenate(chunks, axis=0)

@@ -86,6 +86,13 @@ class GridSarScene(BaseSarScene):
         local = np.stack([xx.reshape(-1), yy.reshape(-1), zz.reshape(-1)], axis=-1)
         return (self.rotation_matrix @ local.T).T + np.expand_dims(self.grid_centre, axis=0)
 
+    def device_positions(self, device=None):
+        """``positions`` generated on the GPU: an [M, 3] float64 torch tensor,
+        bit-identical to the host array (linspace steps as numpy takes them;
+        R @ local as an FMA chain in k order)."""
+        from .._native import grid_positions
+        return grid_positions(self, device)
+
     @staticmethod
     def from_geodetic_enu(grid_shape: Sequence[int], grid_dimensions: Sequence[float],
                           scene_centre_geodetic: NDArray) -> "GridSarScene":

@@ -1,5 +1,8 @@
-"""Oriented bounding boxes of k-space support (reference: utils/k_space.py:9-106)."""
+"""Oriented bounding boxes of k-space support and simple basebanding
+(reference: utils/k_space.py:9-134)."""
 from __future__ import annotations
+
+from typing import Optional
 
 import numpy as np
 from numpy.typing import NDArray
@@ -47,3 +50,36 @@ class OrientedBoundingBox:
 
     def from_obb_coords(self, points: NDArray) -> NDArray:
         return np.dot(self.obb_basis, points.T).T + self.displacement
+
+
+def generate_simple_basebanding_values(scene, aperture, basebanding_spot: Optional[NDArray] = None,
+                                       device=None):
+    """Phases exp(-i sgn (x - spot) . d) that baseband an image about one point,
+    d the displacement of the k-space box of the aperture's band-edge wave
+    vectors seen from the spot (utils/k_space.py:109-134).  The spot defaults
+    to the mean scene position.
+
+    With ``device`` (a torch device), the phases are computed on the GPU and
+    returned as an [M] complex128 torch tensor; a GridSarScene's positions are
+    then generated there and never formed on the host (the default spot is
+    their mean, taken on the device)."""
+    if device is None:
+        pos = scene.positions
+        spot = np.mean(pos, axis=0) if basebanding_spot is None else np.asarray(basebanding_spot)
+        box = OrientedBoundingBox(aperture.edge_k_vecs(spot))
+        phases = (pos - np.expand_dims(spot, axis=0)) @ box.displacement
+        return np.exp(-1.j * aperture.sign_multiplier * phases)
+    import torch
+
+    from .._native import baseband_values, grid_args, grid_positions
+    dev = torch.device(device)
+    if grid_args(scene) is not None:
+        pos = grid_positions(scene, dev)
+    else:
+        pos = torch.as_tensor(np.ascontiguousarray(scene.positions, dtype=np.float64)).to(dev)
+    if basebanding_spot is None:
+        spot = pos.mean(dim=0).cpu().numpy()
+    else:
+        spot = np.asarray(basebanding_spot, dtype=np.float64)
+    box = OrientedBoundingBox(aperture.edge_k_vecs(spot))
+    return baseband_values(pos, spot, box.displacement, float(aperture.sign_multiplier))

@@ -1,0 +1,183 @@
+"""On-device pixel positions (GridSarScene.positions, numpy/hypothesis.py:
+191-206), plans built from them, and basebanding phases
+(utils/k_space.py:109-134) on the GPU, against the host arrays and plans."""
+from decimal import Decimal
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda", 0)
+
+
+AXIS_ROTATIONS = [np.identity(3),
+                  np.array([[0.0, -1.0, 0.0], [1.0, 0.0, 0.0], [0.0, 0.0, 1.0]]),
+                  np.array([[0.0, 0.0, 1.0], [0.0, -1.0, 0.0], [1.0, 0.0, 0.0]])]
+
+
+@pytest.mark.parametrize("shape,dims,centre", [
+    ((1, 1024, 1024), (0.0, 102.3, 102.3), (0.0, 0.0, 0.0)),
+    ((3, 17, 33), (2.0, 5.5, 7.25), (10.0, -3.0, 1.5)),
+    ((1, 1, 1), (0.0, 1.0, 1.0), (4.0, 5.0, 6.0)),
+    ((5, 2, 1), (5e-324, 3.0, 0.0), (0.0, 0.1, 0.2)),
+])
+def test_grid_positions_bit_identical(dev, shape, dims, centre):
+    """Axis-aligned rotations (entries 0, +-1): every rounding order agrees,
+    so the device positions must equal numpy's bit for bit; the cases cover
+    n = 1 axes, zero thickness and numpy's underflowing-step branch."""
+    from sarsolver_amd.numpy import GridSarScene
+    for R in AXIS_ROTATIONS:
+        sc = GridSarScene(shape, dims, grid_centre=np.array(centre), rotation_matrix=R)
+        print(f"case: grid {shape} dims {dims}", flush=True)
+        got = sc.device_positions(dev).cpu().numpy()
+        assert got.shape == (int(np.prod(shape)), 3)
+        assert np.array_equal(got, sc.positions)
+
+
+def test_grid_positions_rotated(dev):
+    """A general rotation (local ENU at a geodetic centre): bit-identical to
+    the FMA chain the device promises (exact decimal arithmetic), and within
+    one ulp of numpy's matmul on this host."""
+    from sarsolver_amd.numpy import GridSarScene
+    sc = GridSarScene.from_geodetic_enu([2, 60, 70], [1.0, 59.0, 69.0],
+                                        np.array([51.5, -1.2, 80.0]))
+    got = sc.device_positions(dev).cpu().numpy()
+    R, c = np.asarray(sc.rotation_matrix), np.asarray(sc.grid_centre)
+    axes = [np.linspace(-0.5 * L, 0.5 * L, n) for L, n in zip(sc.grid_dimensions, sc.grid_shape)]
+
+    def fma(a, b, acc):
+        return float(Decimal(float(a)) * Decimal(float(b)) + Decimal(float(acc)))
+
+    for m in range(0, got.shape[0], 53):
+        ix, iy, iz = m % 70, (m // 70) % 60, m // 4200
+        lx, ly, lz = axes[2][ix], axes[1][iy], axes[0][iz]
+        for r in range(3):
+            v = fma(R[r, 2], lz, fma(R[r, 1], ly, float(R[r, 0]) * float(lx))) + c[r]
+            assert got[m, r] == v
+    host = sc.positions
+    ulp = np.spacing(np.abs(host))
+    assert np.all(np.abs(got - host) <= ulp)
+    print(f"bit-identical to numpy: {np.mean(got == host):.4f} of coordinates")
+
+
+def _plan_inputs(P=256, N=256):
+    from sarsolver_amd import synthetic
+    ap = synthetic.circular_aperture(P, N, 2.0)
+    return ap, (ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns)
+
+
+@pytest.mark.parametrize("interp,precision", [("linear", "series"), ("nearest_ref", "fp64"),
+                                              ("linear", "fp64")])
+def test_grid_and_device_plans_equal_host_plan(dev, oracle_lib, interp, precision):
+    """A plan built from the grid description and one built from a device
+    positions tensor give bit-identical forward and adjoint results to the
+    plan built from host positions (same layout, same kernels); one result is
+    also checked against the FP64 oracle."""
+    from sarsolver_amd._native import SarPlan
+    from sarsolver_amd.numpy import GridSarScene
+    ap, (tx, rx, srp) = _plan_inputs()
+    sc = GridSarScene((1, 96, 80), (0.0, 9.5, 7.9), grid_centre=np.array([1.0, -2.0, 0.0]))
+    W = np.fft.ifftshift(ap.waveform_fft)
+    w = ap.slow_time_weighting
+    args = (W, w, ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, 8.0, 1.0)
+    kw = dict(interp=interp, precision=precision, device=dev)
+    print(f"case: grid/device/host plans {interp} {precision}", flush=True)
+    ph = SarPlan(tx, rx, srp, sc.positions, *args, **kw)
+    pg = SarPlan(tx, rx, srp, sc, *args, **kw)
+    pd = SarPlan(tx, rx, srp, sc.device_positions(dev), *args, **kw)
+    assert pg.position_source == "grid" and pd.position_source == "device"
+    assert ph.layout == pg.layout == pd.layout
+    g = torch.Generator().manual_seed(5)
+    x = torch.complex(torch.randn(sc.num_scatterers, generator=g),
+                      torch.randn(sc.num_scatterers, generator=g)).to(dev)
+    y = torch.complex(torch.randn(256, 256, generator=g), torch.randn(256, 256, generator=g)).to(dev)
+    fh, fg, fd = ph.forward(x), pg.forward(x), pd.forward(x)
+    ah, ag, ad = ph.adjoint(y), pg.adjoint(y), pd.adjoint(y)
+    torch.cuda.synchronize()
+    assert torch.equal(fh, fg) and torch.equal(fh, fd)
+    assert torch.equal(ah, ag) and torch.equal(ah, ad)
+    ref = oracle_lib.adjoint(tx, rx, srp, sc.positions, y.cpu().numpy().astype(np.complex128), W,
+                             w, ap.centre_frequency, ap.sample_bandwidth, upsample_ratio=8.0,
+                             interp=interp)
+    e = rel_l2(ag.cpu().numpy(), ref)
+    print(f"  grid-plan adjoint vs oracle rel L2 {e:.2e}")
+    assert e <= 1e-5
+    assert np.array_equal(pg.geometry["pos"], sc.positions)
+
+
+def test_device_positions_with_nonfinite_pixels(dev):
+    """Non-finite pixels (NaN rows) take the same path from device memory as
+    from host memory: results bit-identical, zero at those pixels."""
+    from sarsolver_amd._native import SarPlan
+    ap, (tx, rx, srp) = _plan_inputs(128, 128)
+    g = np.random.default_rng(2)
+    pos = g.uniform(-5, 5, size=(3000, 3)) * [1, 1, 0.2]
+    pos[[7, 500, 2999]] = np.nan
+    W = np.fft.ifftshift(ap.waveform_fft)
+    args = (W, ap.slow_time_weighting, ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, 8.0,
+            1.0)
+    print("case: device positions with NaN rows", flush=True)
+    ph = SarPlan(tx, rx, srp, pos, *args, interp="linear", precision="series", device=dev)
+    pd = SarPlan(tx, rx, srp, torch.from_numpy(pos).to(dev), *args, interp="linear",
+                 precision="series", device=dev)
+    y = torch.complex(torch.randn(128, 128), torch.randn(128, 128)).to(dev)
+    a1, a2 = ph.adjoint(y), pd.adjoint(y)
+    torch.cuda.synchronize()
+    assert torch.equal(a1, a2)
+    assert torch.all(a2[[7, 500, 2999]] == 0)
+
+
+def test_basebanding_on_device(dev):
+    """Device phases against the host function: with an explicit spot the
+    only difference is FP64 sin/cos (<= 1e-12); with the default spot (mean
+    position, summed in a different order on the device) <= 1e-9."""
+    from sarsolver_amd import synthetic
+    from sarsolver_amd.numpy import GridSarScene, SimpleSarScene
+    from sarsolver_amd.utils import generate_simple_basebanding_values
+    ap = synthetic.circular_aperture(128, 128, 2.0)
+    sc = GridSarScene((1, 300, 200), (0.0, 29.9, 19.9), grid_centre=np.array([2.0, 1.0, 0.0]))
+    spot = np.array([2.5, 0.75, 0.0])
+    print("case: basebanding 300 x 200 grid", flush=True)
+    host = generate_simple_basebanding_values(sc, ap, basebanding_spot=spot)
+    devv = generate_simple_basebanding_values(sc, ap, basebanding_spot=spot, device=dev)
+    assert devv.dtype == torch.complex128
+    assert np.max(np.abs(devv.cpu().numpy() - host)) <= 1e-12
+    host0 = generate_simple_basebanding_values(sc, ap)
+    dev0 = generate_simple_basebanding_values(sc, ap, device=dev).cpu().numpy()
+    assert np.max(np.abs(dev0 - host0)) <= 1e-9
+    pts = SimpleSarScene(sc.positions[::7].copy())
+    d1 = generate_simple_basebanding_values(pts, ap, basebanding_spot=spot, device=dev)
+    h1 = generate_simple_basebanding_values(pts, ap, basebanding_spot=spot)
+    assert np.max(np.abs(d1.cpu().numpy() - h1)) <= 1e-12
+
+
+def test_binding_uses_grid_plans(dev, oracle_lib):
+    """multi_forward/adjoint_evaluate with a GridSarScene build the plan from
+    the grid description (positions never formed on the host) and agree with
+    the same call on a SimpleSarScene of the same positions, bit for bit."""
+    from sarsolver_amd.cxx_binding import _plans, multi_adjoint_evaluate, multi_forward_evaluate
+    from sarsolver_amd.numpy import GridSarScene, SimpleSarScene
+    ap, _ = _plan_inputs(128, 128)
+    sc = GridSarScene((1, 40, 40), (0.0, 3.9, 3.9))
+    pts = SimpleSarScene(sc.positions)
+    _plans.clear_cache()
+    g = np.random.default_rng(1)
+    x = g.standard_normal(1600) + 1j * g.standard_normal(1600)
+    print("case: bindings with a grid scene", flush=True)
+    y1 = multi_forward_evaluate(sc, ap, x, upsample_ratio=8.0)
+    assert any(p.position_source == "grid" for p in _plans._cache.values())
+    y2 = multi_forward_evaluate(pts, ap, x, upsample_ratio=8.0)
+    assert np.array_equal(y1, y2)
+    a1 = multi_adjoint_evaluate(sc, ap, y1, upsample_ratio=8.0)
+    a2 = multi_adjoint_evaluate(pts, ap, y1, upsample_ratio=8.0)
+    assert np.array_equal(a1, a2)
+    _plans.clear_cache()
