@@ -172,14 +172,16 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double 
 
 /* Plan layout: the precision mode in effect (a series request falls back to
  * SARHIP_PRECISION_FP64 when the geometry does not bound the series
- * truncation below 1e-9 bin), whether every valid pulse is monostatic, the
- * number of valid pulses, the forward LDS window length (bins, 0 = global
- * atomics), pulses per forward workgroup, the series truncation bound (bins)
- * and the number of 1024-pixel tiles per forward region.  Any output pointer
- * may be NULL.  No counterpart in the reference. */
+ * truncation below 1e-9 bin and 1e-9 revolution), whether every valid pulse
+ * is monostatic, the number of valid pulses, the forward LDS window length
+ * (bins, 0 = global atomics), pulses per forward workgroup, the series
+ * truncation bound (bins), the number of 1024-pixel tiles per forward region,
+ * and the order of the series the backprojection uses (3 when its bound
+ * meets the same limits, else 5) with its bound.  Any output pointer may be
+ * NULL.  No counterpart in the reference. */
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
                        int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
-                       int *fwd_region_tiles);
+                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins);
 
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
  * (or re-enabling) clears the log; while enabled every forward / adjoint call

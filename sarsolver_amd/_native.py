@@ -94,7 +94,7 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), dp, dp]
         L.sarhip_plan_info.restype = ctypes.c_int
         ip = ctypes.POINTER(ctypes.c_int)
-        L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip]
+        L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip, ip, dp]
         L.sarhip_plan_layout.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
@@ -349,13 +349,17 @@ class SarPlan:
         vals = [ctypes.c_int() for _ in range(5)]
         err = ctypes.c_double()
         reg = ctypes.c_int()
+        bp_ord = ctypes.c_int()
+        bp_err = ctypes.c_double()
         check(L.sarhip_plan_layout(handle, *[ctypes.byref(v) for v in vals], ctypes.byref(err),
-                                   ctypes.byref(reg)), "sarhip_plan_layout")
+                                   ctypes.byref(reg), ctypes.byref(bp_ord), ctypes.byref(bp_err)),
+              "sarhip_plan_layout")
         inv = {v: k for k, v in PRECISION.items()}
         self.layout = {"precision": inv[vals[0].value], "monostatic": bool(vals[1].value),
                        "valid_pulses": vals[2].value, "fwd_window_bins": vals[3].value,
                        "fwd_pulses_per_group": vals[4].value, "series_err_bins": err.value,
-                       "fwd_region_tiles": reg.value}
+                       "fwd_region_tiles": reg.value, "bp_series_order": bp_ord.value,
+                       "bp_series_err_bins": bp_err.value}
         self.effective_precision = self.layout["precision"]
 
     @property

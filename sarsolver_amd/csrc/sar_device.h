@@ -89,8 +89,9 @@ __device__ __forceinline__ double sqrt1pm1_series(const double t) {
 // innermost step is a VOP2 fmac whose accumulator is re-initialised by two
 // v_mov per evaluation.  An identity DPP move (quad_perm [0,1,2,3]) yields
 // the same bits in a register the compiler treats as opaque, computed once.
+// ORD 5: (7/256, -5/128); ORD 3: (1/16, -1/8).
 struct PolyRegs {
-    double c5, c4;
+    double hi, lo;
 };
 __device__ __forceinline__ double opaque_reg(const double v) {
     const long long b = __double_as_longlong(v);
@@ -98,11 +99,17 @@ __device__ __forceinline__ double opaque_reg(const double v) {
     const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0xE4, 0xF, 0xF, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
+template <int ORD = 5>
 __device__ __forceinline__ PolyRegs make_poly_regs() {
-    return PolyRegs{opaque_reg(0.02734375), opaque_reg(-0.0390625)};
+    static_assert(ORD == 3 || ORD == 5, "series order 3 or 5");
+    if constexpr (ORD == 5) return PolyRegs{opaque_reg(0.02734375), opaque_reg(-0.0390625)};
+    else return PolyRegs{opaque_reg(0.0625), opaque_reg(-0.125)};
 }
+// sqrt(1 + t) - 1 truncated after t^ORD; pr from make_poly_regs<ORD>().
+template <int ORD = 5>
 __device__ __forceinline__ double sqrt1pm1_series(const double t, const PolyRegs &pr) {
-    return t * fma(t, fma(t, fma(t, fma(t, pr.c5, pr.c4), 0.0625), -0.125), 0.5);
+    if constexpr (ORD == 5) return t * fma(t, fma(t, fma(t, fma(t, pr.hi, pr.lo), 0.0625), -0.125), 0.5);
+    else return t * fma(t, fma(t, pr.hi, pr.lo), 0.5);
 }
 __device__ __forceinline__ float sqrt1pm1_series_f(const float t) {
     return t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 0.02734375f, -0.0390625f), 0.0625f), -0.125f), 0.5f);
@@ -123,6 +130,14 @@ __device__ __forceinline__ int wrap_bin(const double n, const double nwd, const 
 // is removed in FP64 and the remaining |r| <= 1/2 goes to the hardware
 // v_sin_f32 / v_cos_f32, which take their argument in revolutions (measured
 // on MI355X over [-1/2, 1/2]: max abs error 1.2e-7, rms 3.5e-8).
+// Same for a phase reduced to [0, 1) by v_fract_f64 (one instruction instead
+// of a round and a subtraction; the FP32 value then carries up to 3e-8 rev of
+// rounding instead of 1.5e-8).
+__device__ __forceinline__ void phasor_fract(const double theta, float *s, float *c) {
+    const float r = (float)__builtin_amdgcn_fract(theta);
+    *s = __builtin_amdgcn_sinf(r);
+    *c = __builtin_amdgcn_cosf(r);
+}
 __device__ __forceinline__ void phasor_rev(const double theta, float *s, float *c) {
     const float r = (float)(theta - rint(theta));
     *s = __builtin_amdgcn_sinf(r);

@@ -213,7 +213,7 @@ __device__ __forceinline__ void load_pulse(const KernelParams &k, int tile, int 
 
 // u = dR * by_dx (bins), th = dR * k_c / 2 pi (revolutions).
 // PLANAR: every pixel offset has dz == 0 (the z terms are dropped).
-template <int PREC, bool MONO, bool PLANAR = false>
+template <int PREC, bool MONO, bool PLANAR = false, int SORD = 5>
 __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseData<MONO> &d,
                                            const Pix &q, double &u, double &th,
                                            const PolyRegs &pr) {
@@ -225,7 +225,7 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
         if constexpr (MONO) {
             const double t = PLANAR ? fma(d.s.bx, q.dx, fma(d.s.by, q.dy, d.s.c * q.d2))
                                     : fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
-            const double g = sqrt1pm1_series(t, pr);
+            const double g = sqrt1pm1_series<SORD>(t, pr);
             u = fma(d.s.ku, g, d.s.cu);
             th = fma(d.s.kt, g, d.s.ct);
         } else {
@@ -233,7 +233,7 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
             const double zr = PLANAR ? d.s.rc * q.d2 : fma(d.s.rbz, q.dz, d.s.rc * q.d2);
             const double tt = fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, zt));
             const double tr = fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, zr));
-            const double gt = sqrt1pm1_series(tt, pr), gr = sqrt1pm1_series(tr, pr);
+            const double gt = sqrt1pm1_series<SORD>(tt, pr), gr = sqrt1pm1_series<SORD>(tr, pr);
             u = fma(d.s.kut, gt, fma(d.s.kur, gr, d.s.cu));
             th = fma(d.s.ktt, gt, fma(d.s.ktr, gr, d.s.ct));
         }
@@ -316,10 +316,10 @@ __device__ __forceinline__ int bin_index(const KernelParams &k, double n, int sp
 // sums in FP32; every 64 pulses the per-pixel FP32 sums are folded into FP64
 // sums kept in LDS (out of registers, for occupancy).
 // ---------------------------------------------------------------------------
-template <int INTERP, bool MONO, int PREC, bool PLANAR>
+template <int INTERP, bool MONO, int PREC, bool PLANAR, int SORD = 5>
 __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
     __shared__ double acc[2 * kPixPerThread][kThreads];
-    const PolyRegs pr = make_poly_regs();
+    const PolyRegs pr = make_poly_regs<SORD>();
     // grid: (tile, pulse split); split s takes the valid pulses [j_begin, j_end)
     const int tile = blockIdx.x;
     const int split = blockIdx.y, nsplit = (int)gridDim.y;
@@ -346,23 +346,29 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
             __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0, k.ld * 8, kBufferWord3);
         constexpr bool kShift = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
         if constexpr (kShift) {
-            // Move u by a multiple of nw so that every pixel of the tile lands
-            // in [0, nw + 2 shift_lo): u(tile centre) - shift_lo is brought
-            // into [0, nw).  Bins are unchanged modulo nw and the padded row
-            // tail makes bins n + 1 <= ld - 1 valid without a wrap.
-            d.s.cu = fma(-floor((d.s.cu - k.shift_lo) * k.inv_nw), k.nwd, d.s.cu);
+            // u of the tile centre moved by a multiple of nw (bp_shift_kernel,
+            // once per plan) so that every pixel of the tile lands in
+            // [2, nw + 2 shift_lo - 2): bins are unchanged modulo nw and the
+            // padded row tail makes bins n + 1 <= ld - 1 valid without a wrap.
+            d.s.cu = k.bp_cu[(size_t)tile * k.Pv + j];
         }
 #pragma unroll
         for (int m = 0; m < kPixPerThread; ++m) {
             double u, th;
-            eval_range<PREC, MONO, PLANAR>(k, d, q[m], u, th, pr);
+            eval_range<PREC, MONO, PLANAR, SORD>(k, d, q[m], u, th, pr);
             float hr, hi;
             if constexpr (INTERP == kInterpLinear) {
-                const double fl = floor(u);
-                const float fr = (float)(u - fl);
                 int n0;
-                if constexpr (kShift) n0 = (int)fl;
-                else n0 = bin_index(k, fl, 1);
+                float fr;
+                if constexpr (kShift) {
+                    // u > 0: truncation is floor, v_fract_f64 the exact rest
+                    n0 = (int)u;
+                    fr = (float)__builtin_amdgcn_fract(u);
+                } else {
+                    const double fl = floor(u);
+                    fr = (float)(u - fl);
+                    n0 = bin_index(k, fl, 1);
+                }
                 const u32x4 h = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
                 const float h0x = __uint_as_float(h.x), h0y = __uint_as_float(h.y);
                 hr = fmaf(fr, __uint_as_float(h.z) - h0x, h0x);
@@ -374,7 +380,8 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
                 hi = h.y;
             }
             float s, c;
-            phasor_rev(th, &s, &c);
+            if constexpr (kShift) phasor_fract(th, &s, &c);
+            else phasor_rev(th, &s, &c);
             // h * e^{-i theta}
             ar[m] = fmaf(hr, c, fmaf(hi, s, ar[m]));
             ai[m] = fmaf(hi, c, fmaf(-hr, s, ai[m]));
@@ -849,6 +856,28 @@ hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 
     return hipGetLastError();
 }
 
+// out[i] = cu_i moved by a multiple of nw into [shift_lo, shift_lo + nw):
+// the backprojection's row offset of the tile centre for (tile, pulse) i,
+// cu_i the `cu` field of that record (index cu_at of `stride` doubles).
+__global__ __launch_bounds__(256) void bp_shift_kernel(const double *__restrict__ series,
+                                                       int stride, int cu_at, size_t n,
+                                                       double shift_lo, double inv_nw, double nwd,
+                                                       double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const double cu = series[i * stride + cu_at];
+    out[i] = fma(-floor((cu - shift_lo) * inv_nw), nwd, cu);
+}
+
+hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
+                           double nwd, double *out, hipStream_t s) {
+    const int stride = mono ? (int)(sizeof(SeriesMono) / 8) : (int)(sizeof(SeriesBi) / 8);
+    const int cu_at = mono ? (int)(offsetof(SeriesMono, cu) / 8) : (int)(offsetof(SeriesBi, cu) / 8);
+    bp_shift_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>((const double *)series, stride,
+                                                               cu_at, n, shift_lo, inv_nw, nwd, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const PulseConst *pc,
                                 const int *rows, int Pv, double by_dx, double kc_rev, bool mono,
                                 void *out, hipStream_t s) {
@@ -870,8 +899,13 @@ static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const bool pl = I == kInterpLinear && k.planar;
     switch (k.precision) {
         case kPrecSeries:
-            if (pl) bp_gather_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, 0, s>>>(k);
-            else bp_gather_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, 0, s>>>(k);
+            if (I == kInterpLinear && k.series_order == 3) {
+                if (pl) bp_gather_kernel<I, MO, kPrecSeries, true, 3><<<grid, kThreads, 0, s>>>(k);
+                else bp_gather_kernel<I, MO, kPrecSeries, false, 3><<<grid, kThreads, 0, s>>>(k);
+            } else {
+                if (pl) bp_gather_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, 0, s>>>(k);
+                else bp_gather_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, 0, s>>>(k);
+            }
             break;
         case kPrecHilo:
             if (pl) bp_gather_kernel<I, MO, kPrecHilo, true><<<grid, kThreads, 0, s>>>(k);

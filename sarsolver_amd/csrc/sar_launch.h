@@ -47,6 +47,8 @@ struct KernelParams {
     float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
+    const double *bp_cu;      // [num_tiles][Pv] series `cu` moved into [shift_lo, shift_lo + nw)
+    int series_order;         // backprojection series order (3 or 5; forward always 5)
     // variant
     int interp, precision;
     bool mono;
@@ -92,6 +94,8 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
                                 const int *rows, int Pv, double by_dx, double kc_rev, bool mono,
                                 void *out, hipStream_t s);
 hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
+hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
+                           double nwd, double *out, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)
 hipError_t launch_rs_ramp(const float2 *in, int P, int N, const double *dr, double fs_over_n,
                           double sgn_fc, double inv_c, float2 *out, hipStream_t s);

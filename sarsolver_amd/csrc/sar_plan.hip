@@ -114,6 +114,8 @@ struct sarhip_plan_s {
     int num_tiles = 0, win = 0, reduce_chunk = 0;
     int num_regions = 0, region_tiles = 1, fwd_pulses = 32, fwd_groups = 1;
     double series_err_bins = 0;  // bound on the series truncation term (bins)
+    double bp_series_err_bins = 0;  // the same for the backprojection's series order
+    int series_order = 5;           // backprojection series order (3 or 5)
     size_t ld = 0;               // adjoint profile row stride (nw + padded tail)
     double shift_lo = 0;         // bins: bound on |u - u(tile centre)| plus one
 
@@ -124,6 +126,7 @@ struct sarhip_plan_s {
     double4 *d_tile_c = nullptr, *d_region_c = nullptr, *d_region_h = nullptr;
     PixRec *d_pixrec = nullptr;
     void *d_series = nullptr;
+    double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
     float2 *d_amps_perm = nullptr;
@@ -158,6 +161,7 @@ struct sarhip_plan_s {
         dev_free(d_region_h);
         dev_free(d_pixrec);
         dev_free(d_series);
+        dev_free(d_bp_cu);
         dev_free(d_W);
         dev_free(d_Wconj);
         dev_free(d_wfac_fwd);
@@ -192,6 +196,8 @@ struct sarhip_plan_s {
         k.rows = d_rows;
         k.Pv = Pv;
         k.series = d_series;
+        k.bp_cu = d_bp_cu;
+        k.series_order = series_order;
         k.prof = d_prof;
         k.nw = (int)nw;
         k.ld = (int)ld;
@@ -292,8 +298,13 @@ double window_bins(const std::vector<double4> &reg, const std::vector<double4> &
 
 // Largest truncation term of the 5th-order series, 21/1024 t^6 per leg, in
 // bins, over every (tile, valid pulse); t <= (2 A r + r^2) / A^2.
+// Bound (bins) on the truncation of sqrt(1 + t) - 1 after t^order over every
+// (tile, pulse): |t| <= (2 A r + r^2) / A^2 for a tile of radius r at range A,
+// and the remainder is at most the first omitted term / (1 - |t|) (the terms
+// alternate for t > 0 and shrink geometrically for t < 0).
 double series_error_bound(const std::vector<double4> &tiles, const std::vector<PulseConst> &pc,
-                          const std::vector<int> &rows, bool mono, double by_dx) {
+                          const std::vector<int> &rows, bool mono, double by_dx, int order) {
+    const double coef = order == 3 ? 5.0 / 128.0 : 21.0 / 1024.0;
     double worst = 0.0;
     for (const double4 &tl : tiles) {
         const double r = tl.w;
@@ -305,7 +316,8 @@ double series_error_bound(const std::vector<double4> &tiles, const std::vector<P
                 const double A = std::sqrt(dx * dx + dy * dy + dz * dz);
                 if (!(A > 2.0 * r)) return INFINITY;
                 const double t = (2.0 * A * r + r * r) / (A * A);
-                legs += (21.0 / 1024.0) * std::pow(t, 6) * A * (mono ? 2.0 : 1.0);
+                if (!(t < 0.5)) return INFINITY;
+                legs += coef * std::pow(t, order + 1) / (1.0 - t) * A * (mono ? 2.0 : 1.0);
             }
             worst = std::max(worst, legs * by_dx);
         }
@@ -655,13 +667,26 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     // geometry does not allow it the series mode falls back to exact FP64.
     pl->requested_precision = precision;
     pl->precision = precision;
+    // The backprojection of a linear series plan uses the third-order series
+    // when its bound meets the same limits (the forward keeps the fifth).
+    pl->series_order = 5;
     if (precision == SARHIP_PRECISION_SERIES || precision == SARHIP_PRECISION_HILO) {
-        pl->series_err_bins = series_error_bound(tiles, pc, rows, mono, pl->by_dx);
-        const double phase_err = pl->series_err_bins * pl->kc_rev / pl->by_dx;
-        if (precision == SARHIP_PRECISION_SERIES &&
-            !(pl->series_err_bins <= 1e-9 && std::fabs(phase_err) <= 1e-9))
+        auto ok = [&](double bins) {
+            return bins <= 1e-9 && std::fabs(bins * pl->kc_rev / pl->by_dx) <= 1e-9;
+        };
+        pl->series_err_bins = series_error_bound(tiles, pc, rows, mono, pl->by_dx, 5);
+        if (precision == SARHIP_PRECISION_SERIES && !ok(pl->series_err_bins))
             pl->precision = SARHIP_PRECISION_FP64;
+        if (pl->precision == SARHIP_PRECISION_SERIES && interp == SARHIP_INTERP_LINEAR &&
+            env_int("SARHIP_SERIES_ORDER", 3) == 3) {
+            const double e3 = series_error_bound(tiles, pc, rows, mono, pl->by_dx, 3);
+            if (ok(e3)) {
+                pl->series_order = 3;
+                pl->bp_series_err_bins = e3;
+            }
+        }
     }
+    if (pl->series_order == 5) pl->bp_series_err_bins = pl->series_err_bins;
     const bool need_series =
         pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO;
 
@@ -777,6 +802,12 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         HIP_TRY(hipMalloc(&pl->d_series, rec * pl->num_tiles * (size_t)pl->Pv));
         HIP_TRY(launch_series_consts(pl->d_tile_c, pl->num_tiles, pl->d_pc, pl->d_rows, pl->Pv,
                                      pl->by_dx, pl->kc_rev, mono, pl->d_series, nullptr));
+        if (interp == SARHIP_INTERP_LINEAR) {
+            const size_t n = (size_t)pl->num_tiles * (size_t)pl->Pv;
+            HIP_TRY(hipMalloc((void **)&pl->d_bp_cu, sizeof(double) * n));
+            HIP_TRY(launch_bp_shift(pl->d_series, mono, n, pl->shift_lo, pl->inv_nw, (double)nw,
+                                    pl->d_bp_cu, nullptr));
+        }
     }
 
     // rocFFT: batched in-place complex64 transforms of length N_w.
@@ -925,8 +956,10 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *nw, double *by_dx, double *kc) 
 
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
                        int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
-                       int *fwd_region_tiles) {
+                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins) {
     if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
+    if (bp_series_order) *bp_series_order = plan->series_order;
+    if (bp_series_err_bins) *bp_series_err_bins = plan->bp_series_err_bins;
     if (fwd_region_tiles) *fwd_region_tiles = plan->region_tiles;
     if (precision) *precision = plan->precision;
     if (monostatic) *monostatic = plan->mono ? 1 : 0;
