@@ -176,8 +176,8 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double 
  * is monostatic, the number of valid pulses, the forward LDS window length
  * (bins, 0 = global atomics), pulses per forward workgroup, the series
  * truncation bound (bins), the number of 1024-pixel tiles per forward region,
- * and the order of the series the backprojection uses (3 when its bound
- * meets the same limits, else 5) with its bound.  Any output pointer may be
+ * and the order of the series the linear-interpolation kernels use (3 when
+ * its bound meets the same limits, else 5) with its bound.  Any output pointer may be
  * NULL.  No counterpart in the reference. */
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
                        int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,

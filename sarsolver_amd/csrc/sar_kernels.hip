@@ -528,7 +528,7 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict_
     }
 }
 
-template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT>
+template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT, int SORD = 5>
 __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     // Fixed-point windows [G][wst], one 64-bit word per bin holding
     // im * 2^32 + re: a single 64-bit LDS add accumulates both components
@@ -560,7 +560,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const int t0 = region * R;
     const int t1 = min(k.num_tiles, t0 + R);
 
-    const PolyRegs pr = make_poly_regs();
+    const PolyRegs pr = make_poly_regs<SORD>();
     // region L1 norm (region_l1_kernel) -> fixed-point scale
     const float l1 = k.region_l1[region];
     const bool fixed_ok = isfinite(l1) && win > 0;
@@ -625,7 +625,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
                         a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + i + nstreams));
                     }
-                    if (r.ok == 0.0) continue;
+                    // ok is exactly 0.0 or 1.0: an integer test keeps it on the scalar unit
+                    if (__double_as_longlong(r.ok) == 0) continue;
                 } else {
                     r = k.pixrec[tb + i];
                     if (r.ok == 0.0) continue;
@@ -641,10 +642,14 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                 px.d2 = r.d2;
                 px.ok = true;
                 double u, th;
-                eval_range<PREC, MONO, PLANAR>(k, d, px, u, th, pr);
+                eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
                 float sn, cs;
                 double fl;
-                if constexpr (INTERP == kInterpLinear) {
+                if constexpr (INTERP == kInterpLinear && kRel) {
+                    // u is taken from the window base, at least 2 bins below
+                    // every pixel of the region: truncation is floor
+                    fl = (double)(int)u;
+                } else if constexpr (INTERP == kInterpLinear) {
                     fl = floor(u);
                 } else {
                     // forward bin rule round(u) for both nearest modes (CPP:289)
@@ -656,7 +661,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                 const float ci = fmaf(cs, a.y, sn * a.x);
                 const int l = kRel ? (int)fl : (int)(fl - wb);
                 if constexpr (INTERP == kInterpLinear) {
-                    const float fr = (float)(u - fl);
+                    const float fr = kRel ? (float)__builtin_amdgcn_fract(u) : (float)(u - fl);
                     if (fixed_ok && (unsigned)l < (unsigned)(win - 1)) {
                         if (qok) {
                             const int tr = fix_rn(cr), ti = fix_rn(ci);
@@ -666,7 +671,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         }
                     } else if (qok) {
                         const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
-                        const int n0 = bin_index(k, kRel ? fl + wb : fl, 1);
+                        const int n0 = bin_index(k, kRel ? floor(u) + wb : fl, 1);
                         const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
                         atomicAdd(&row[n0].x, gr * cr);
                         atomicAdd(&row[n0].y, gr * ci);
@@ -940,8 +945,13 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const bool pl = k.planar;
     switch (k.precision) {
         case kPrecSeries:
-            if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true, NT><<<grid, NT, lds, s>>>(k);
-            else fp_scatter_kernel<I, MO, kPrecSeries, false, NT><<<grid, NT, lds, s>>>(k);
+            if (I == kInterpLinear && k.series_order == 3) {
+                if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true, NT, 3><<<grid, NT, lds, s>>>(k);
+                else fp_scatter_kernel<I, MO, kPrecSeries, false, NT, 3><<<grid, NT, lds, s>>>(k);
+            } else {
+                if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true, NT><<<grid, NT, lds, s>>>(k);
+                else fp_scatter_kernel<I, MO, kPrecSeries, false, NT><<<grid, NT, lds, s>>>(k);
+            }
             break;
         case kPrecHilo:
             if (pl) fp_scatter_kernel<I, MO, kPrecHilo, true, NT><<<grid, NT, lds, s>>>(k);

@@ -48,7 +48,7 @@ struct KernelParams {
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
     const double *bp_cu;      // [num_tiles][Pv] series `cu` moved into [shift_lo, shift_lo + nw)
-    int series_order;         // backprojection series order (3 or 5; forward always 5)
+    int series_order;         // series order of the linear gather / scatter (3 or 5)
     // variant
     int interp, precision;
     bool mono;

@@ -114,8 +114,8 @@ struct sarhip_plan_s {
     int num_tiles = 0, win = 0, reduce_chunk = 0;
     int num_regions = 0, region_tiles = 1, fwd_pulses = 32, fwd_groups = 1;
     double series_err_bins = 0;  // bound on the series truncation term (bins)
-    double bp_series_err_bins = 0;  // the same for the backprojection's series order
-    int series_order = 5;           // backprojection series order (3 or 5)
+    double bp_series_err_bins = 0;  // the same for the linear kernels' series order
+    int series_order = 5;           // series order of the linear kernels (3 or 5)
     size_t ld = 0;               // adjoint profile row stride (nw + padded tail)
     double shift_lo = 0;         // bins: bound on |u - u(tile centre)| plus one
 
@@ -667,8 +667,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     // geometry does not allow it the series mode falls back to exact FP64.
     pl->requested_precision = precision;
     pl->precision = precision;
-    // The backprojection of a linear series plan uses the third-order series
-    // when its bound meets the same limits (the forward keeps the fifth).
+    // The linear kernels of a series plan use the third-order series when its
+    // bound meets the same limits (the nearest modes keep the fifth).
     pl->series_order = 5;
     if (precision == SARHIP_PRECISION_SERIES || precision == SARHIP_PRECISION_HILO) {
         auto ok = [&](double bins) {
