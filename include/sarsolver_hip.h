@@ -150,6 +150,15 @@ int sarhip_baseband_values(const void *d_pos, size_t num_scatterers, const doubl
                            const double displacement[3], double sign_multiplier, void *d_out,
                            void *stream);
 
+/* Decode a CPHD signal array on the device, replacing the host decode inside
+ * sarpy's reader that io/cphd101.py:252-267 (io/cphd03.py) calls: d_raw holds
+ * num_vectors x num_samples big-endian complex samples (format 0 = CF8 /
+ * RE32F_IM32F, 1 = CI4 / RE16I_IM16I, 2 = CI2 / RE08I_IM08I), aligned to one
+ * sample; d_amp_sf [num_vectors] float32 per-vector scale factors or NULL;
+ * d_out [num_vectors][num_samples] complex64.  Stream-ordered on `stream`. */
+int sarhip_decode_signal(const void *d_raw, int format, size_t num_vectors, size_t num_samples,
+                         const float *d_amp_sf, void *d_out, void *stream);
+
 int sarhip_plan_destroy(sarhip_plan_t plan);
 
 /* Forward projection on device buffers: amps [M] complex64 ->

@@ -51,7 +51,7 @@ EXPORTED_SYMBOLS = (
     "sarhip_plan_set_timing",
     "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
     "sarhip_resample", "sarhip_plan_create_device_positions", "sarhip_plan_create_grid",
-    "sarhip_grid_positions", "sarhip_baseband_values",
+    "sarhip_grid_positions", "sarhip_baseband_values", "sarhip_decode_signal",
     "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
     "sarhip_conv_last_error",
 )
@@ -122,6 +122,8 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_grid_positions.restype = ctypes.c_int
         L.sarhip_baseband_values.argtypes = [vp, sz, dp, dp, dbl, vp, vp]
         L.sarhip_baseband_values.restype = ctypes.c_int
+        L.sarhip_decode_signal.argtypes = [vp, ctypes.c_int, sz, sz, vp, vp, vp]
+        L.sarhip_decode_signal.restype = ctypes.c_int
         _lib = L
         return L
 
@@ -251,6 +253,29 @@ def baseband_values(positions, spot, displacement, sign_multiplier: float):
                                                     ctypes.c_void_p(out.data_ptr()),
                                                     _stream_handle(pos.device))
     check(rc, "sarhip_baseband_values")
+    return out
+
+
+def decode_signal(raw: np.ndarray, format_code: int, num_samples: int, amp_sf=None, device=None):
+    """Raw big-endian CPHD samples (uint8 [n, num_samples * bytes per sample],
+    host) decoded on the GPU into an [n, num_samples] complex64 tensor;
+    amp_sf [n] scales each vector when given."""
+    torch = _torch()
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    raw = np.ascontiguousarray(raw, dtype=np.uint8)
+    n = raw.shape[0] if raw.ndim == 2 else 0
+    out = torch.empty((n, int(num_samples)), dtype=torch.complex64, device=dev)
+    if n == 0:
+        return out
+    d_raw = torch.from_numpy(raw.reshape(-1)).to(dev)
+    d_sf = None if amp_sf is None else torch.from_numpy(
+        np.ascontiguousarray(amp_sf, dtype=np.float32)).to(dev)
+    with torch.cuda.device(dev):
+        rc = load_library().sarhip_decode_signal(
+            ctypes.c_void_p(d_raw.data_ptr()), int(format_code), n, int(num_samples),
+            ctypes.c_void_p(d_sf.data_ptr()) if d_sf is not None else None,
+            ctypes.c_void_p(out.data_ptr()), _stream_handle(dev))
+    check(rc, "sarhip_decode_signal")
     return out
 
 

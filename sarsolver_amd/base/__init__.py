@@ -1,2 +1,2 @@
 from .hypothesis import block_hypothesis_vector, BaseSarScene
-from .measurement import BaseSimpleSarAperture, block_phase_history_array
+from .measurement import BaseSimpleSarAperture, BaseSimpleSarDataset, block_phase_history_array

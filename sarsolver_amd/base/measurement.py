@@ -129,3 +129,21 @@ class BaseSimpleSarAperture(ABC):
     @abstractmethod
     def get_subset(self, item: slice = None) -> "BaseSimpleSarAperture":
         raise NotImplementedError
+
+
+class BaseSimpleSarDataset(ABC):
+    """An aperture with its measured phase history (base/measurement.py:358-402)."""
+
+    @property
+    @abstractmethod
+    def aperture(self) -> BaseSimpleSarAperture:
+        raise NotImplementedError
+
+    @abstractmethod
+    def phase_history(self, item: slice = slice(None)) -> Optional[NDArray]:
+        """[num_requested, num_freqs] complex phase history (centred order)."""
+        raise NotImplementedError
+
+    @abstractmethod
+    def get_subset(self, item: slice = None) -> "BaseSimpleSarDataset":
+        raise NotImplementedError

@@ -14,7 +14,8 @@
 //   rs_window_kernel : crop N2 samples, slow-time kernel over each
 //                      window, Tukey taper / N, fftshift phase         [P2][N2]
 //   rocFFT forward, length N2, P2 rows (in place, on the output)
-// Both kernels are HBM-bound elementwise passes.
+// Both kernels are HBM-bound elementwise passes.  decode_signal_kernel turns
+// the raw big-endian samples of a CPHD signal array into complex64.
 #include <hip/hip_runtime.h>
 
 #include "sar_device.h"
@@ -82,6 +83,39 @@ __global__ __launch_bounds__(256) void rs_window_kernel(const float2 *__restrict
     out[(size_t)i * N2 + j] = cmulf(make_float2(acc.x * wj, acc.y * wj), ph);
 }
 
+// CPHD signal samples (big-endian) to complex64, times the vector's amplitude
+// scale when given: format 0 = two float32 ("CF8"), 1 = two int16 ("CI4"),
+// 2 = two int8 ("CI2").  One thread per sample of row p.
+__global__ __launch_bounds__(256) void decode_signal_kernel(const unsigned char *__restrict__ raw,
+                                                            int format, int ns,
+                                                            const float *__restrict__ sf,
+                                                            float2 *__restrict__ out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (j >= ns) return;
+    const size_t i = (size_t)p * ns + j;
+    float re, im;
+    if (format == 0) {
+        const uint2 v = ((const uint2 *)raw)[i];
+        re = __uint_as_float(__builtin_bswap32(v.x));
+        im = __uint_as_float(__builtin_bswap32(v.y));
+    } else if (format == 1) {
+        const unsigned v = ((const unsigned *)raw)[i];
+        re = (float)(short)(((v & 0xffu) << 8) | ((v >> 8) & 0xffu));
+        im = (float)(short)((((v >> 16) & 0xffu) << 8) | (v >> 24));
+    } else {
+        const unsigned short v = ((const unsigned short *)raw)[i];
+        re = (float)(signed char)(v & 0xffu);
+        im = (float)(signed char)(v >> 8);
+    }
+    if (sf) {
+        const float a = sf[p];
+        re *= a;
+        im *= a;
+    }
+    out[i] = make_float2(re, im);
+}
+
 }  // namespace
 
 // Rows go on grid y, whose extent is at most 65535: launch in chunks of rows.
@@ -103,6 +137,18 @@ hipError_t launch_rs_window(const float2 *prof, int N, int N2, const float *win,
         const int rows = P2 - i0 < kMaxRows ? P2 - i0 : kMaxRows;
         rs_window_kernel<<<dim3((N2 + 255) / 256, rows), 256, 0, s>>>(
             prof, N, N2, win, kern, K, starts + i0, out + (size_t)i0 * N2);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, const float *sf,
+                                float2 *out, hipStream_t s) {
+    const size_t row = (size_t)ns * (format == 0 ? 8 : format == 1 ? 4 : 2);
+    for (int p0 = 0; p0 < nv; p0 += kMaxRows) {
+        const int rows = nv - p0 < kMaxRows ? nv - p0 : kMaxRows;
+        decode_signal_kernel<<<dim3((ns + 255) / 256, rows), 256, 0, s>>>(
+            (const unsigned char *)raw + (size_t)p0 * row, format, ns, sf ? sf + p0 : nullptr,
+            out + (size_t)p0 * ns);
     }
     return hipGetLastError();
 }

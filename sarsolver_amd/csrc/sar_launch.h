@@ -101,6 +101,8 @@ hipError_t launch_rs_ramp(const float2 *in, int P, int N, const double *dr, doub
                           double sgn_fc, double inv_c, float2 *out, hipStream_t s);
 hipError_t launch_rs_window(const float2 *prof, int N, int N2, const float *win, const float2 *kern,
                             int K, const int *starts, int P2, float2 *out, hipStream_t s);
+hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, const float *sf,
+                                float2 *out, hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

@@ -931,6 +931,20 @@ int sarhip_grid_positions(const size_t grid_shape[3], const double grid_dimensio
     return SARHIP_OK;
 }
 
+int sarhip_decode_signal(const void *d_raw, int format, size_t num_vectors, size_t num_samples,
+                         const float *d_amp_sf, void *d_out, void *stream) {
+    if (!d_raw || !d_out) return fail(SARHIP_EINVAL, "NULL argument");
+    if (format < 0 || format > 2) return fail(SARHIP_EINVAL, "format must be 0 (CF8), 1 (CI4) or 2 (CI2)");
+    if (num_vectors > (size_t)INT32_MAX || num_samples > (size_t)INT32_MAX / 8)
+        return fail(SARHIP_EINVAL, "signal array too large");
+    if (num_vectors == 0 || num_samples == 0) return SARHIP_OK;
+    const size_t align = format == 0 ? 8 : format == 1 ? 4 : 2;
+    if ((uintptr_t)d_raw % align) return fail(SARHIP_EINVAL, "raw samples must be aligned to one sample");
+    HIP_TRY(launch_decode_signal(d_raw, format, (int)num_vectors, (int)num_samples, d_amp_sf,
+                                 (float2 *)d_out, (hipStream_t)stream));
+    return SARHIP_OK;
+}
+
 int sarhip_baseband_values(const void *d_pos, size_t num_scatterers, const double spot[3],
                            const double displacement[3], double sign_multiplier, void *d_out,
                            void *stream) {

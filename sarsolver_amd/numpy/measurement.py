@@ -7,7 +7,7 @@ import numpy as np
 from numpy.typing import NDArray
 from scipy.constants import c
 
-from ..base import BaseSimpleSarAperture
+from ..base import BaseSimpleSarAperture, BaseSimpleSarDataset
 
 
 class NumpySimpleSarAperture(BaseSimpleSarAperture):
@@ -112,7 +112,7 @@ class NumpySimpleSarAperture(BaseSimpleSarAperture):
                                       sign_multiplier=self.sign_multiplier, c_eff=self.c_eff)
 
 
-class NumpySimpleSarDataset:
+class NumpySimpleSarDataset(BaseSimpleSarDataset):
     """An aperture plus its measured phase history [P, N] (centred order)
     (numpy/measurement.py:244-501)."""
 
@@ -207,7 +207,11 @@ class NumpySimpleSarDataset:
         dr = srp_range_change(srps, new_srp, tx, rx)
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
-        ph = original_dataset.phase_history()
+        if hasattr(original_dataset, "phase_history_device"):
+            # file-backed datasets decode their raw samples on the GPU
+            ph = original_dataset.phase_history_device(slice(None), device)
+        else:
+            ph = original_dataset.phase_history()
         if isinstance(ph, torch.Tensor):
             ph_dev = ph.to(device=device, dtype=torch.complex64)
         else:
