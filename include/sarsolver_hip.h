@@ -185,12 +185,14 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double 
  * is monostatic, the number of valid pulses, the forward LDS window length
  * (bins, 0 = global atomics), pulses per forward workgroup, the series
  * truncation bound (bins), the number of 1024-pixel tiles per forward region,
- * and the order of the series the linear-interpolation kernels use (3 when
- * its bound meets the same limits, else 5) with its bound.  Any output pointer may be
- * NULL.  No counterpart in the reference. */
+ * the order of the series the linear-interpolation kernels use (3 when its
+ * bound meets the same limits, else 5) with its bound, and the upsampling
+ * ratio U of the fused spectral kernels (N_w = U N; 0 = rocFFT stages).  Any
+ * output pointer may be NULL.  No counterpart in the reference. */
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
                        int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
-                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins);
+                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins,
+                       int *fused_fft_ratio);
 
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
  * (or re-enabling) clears the log; while enabled every forward / adjoint call
@@ -202,6 +204,11 @@ int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int 
  * the reference (it has no instrumentation). */
 int sarhip_plan_set_timing(sarhip_plan_t plan, int enable);
 int sarhip_plan_timing(sarhip_plan_t plan, int which, float *ms4, int *ncalls);
+/* Mean device time (ms) of the main kernel alone over the logged calls of
+ * `which`: the scatter launch of the forward (without the region norms and
+ * the window merge) or the gather launch of the adjoint (without the
+ * pulse-split sum).  No counterpart in the reference. */
+int sarhip_plan_kernel_timing(sarhip_plan_t plan, int which, float *ms, int *ncalls);
 
 /* Message describing the most recent failure on the calling thread. */
 const char *sarhip_last_error(void);

@@ -49,7 +49,7 @@ EXPORTED_SYMBOLS = (
     "forward_evaluate", "adjoint_evaluate", "sarhip_plan_create", "sarhip_plan_destroy",
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
     "sarhip_plan_set_timing",
-    "sarhip_plan_timing", "sarhip_last_error", "sarhip_version",
+    "sarhip_plan_timing", "sarhip_plan_kernel_timing", "sarhip_last_error", "sarhip_version",
     "sarhip_resample", "sarhip_plan_create_device_positions", "sarhip_plan_create_grid",
     "sarhip_grid_positions", "sarhip_baseband_values", "sarhip_decode_signal",
     "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
@@ -94,13 +94,16 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_plan_info.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t), dp, dp]
         L.sarhip_plan_info.restype = ctypes.c_int
         ip = ctypes.POINTER(ctypes.c_int)
-        L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip, ip, dp]
+        L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip, ip, dp, ip]
         L.sarhip_plan_layout.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
         L.sarhip_plan_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                          ctypes.POINTER(ctypes.c_int)]
         L.sarhip_plan_timing.restype = ctypes.c_int
+        L.sarhip_plan_kernel_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                                ctypes.POINTER(ctypes.c_int)]
+        L.sarhip_plan_kernel_timing.restype = ctypes.c_int
         L.sarhip_last_error.argtypes = []
         L.sarhip_last_error.restype = ctypes.c_char_p
         L.sarhip_version.argtypes = []
@@ -376,15 +379,17 @@ class SarPlan:
         reg = ctypes.c_int()
         bp_ord = ctypes.c_int()
         bp_err = ctypes.c_double()
+        fused = ctypes.c_int()
         check(L.sarhip_plan_layout(handle, *[ctypes.byref(v) for v in vals], ctypes.byref(err),
-                                   ctypes.byref(reg), ctypes.byref(bp_ord), ctypes.byref(bp_err)),
+                                   ctypes.byref(reg), ctypes.byref(bp_ord), ctypes.byref(bp_err),
+                                   ctypes.byref(fused)),
               "sarhip_plan_layout")
         inv = {v: k for k, v in PRECISION.items()}
         self.layout = {"precision": inv[vals[0].value], "monostatic": bool(vals[1].value),
                        "valid_pulses": vals[2].value, "fwd_window_bins": vals[3].value,
                        "fwd_pulses_per_group": vals[4].value, "series_err_bins": err.value,
                        "fwd_region_tiles": reg.value, "bp_series_order": bp_ord.value,
-                       "bp_series_err_bins": bp_err.value}
+                       "bp_series_err_bins": bp_err.value, "fused_fft_ratio": fused.value}
         self.effective_precision = self.layout["precision"]
 
     @property
@@ -467,8 +472,12 @@ class SarPlan:
         n = ctypes.c_int()
         check(load_library().sarhip_plan_timing(self._handle, {"forward": 0, "adjoint": 1}[which],
                                                 buf, ctypes.byref(n)), "sarhip_plan_timing")
+        ms = ctypes.c_float()
+        check(load_library().sarhip_plan_kernel_timing(
+            self._handle, {"forward": 0, "adjoint": 1}[which], ctypes.byref(ms), None),
+            "sarhip_plan_kernel_timing")
         return {"window": buf[0], "fft": buf[1], "kernel": buf[2], "total": buf[3],
-                "calls": n.value}
+                "main_kernel": ms.value, "calls": n.value}
 
 
 def resample(phase_history, dr, centre_freq: float, sample_freq: float, sign_multiplier: float,

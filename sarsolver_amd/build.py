@@ -17,7 +17,7 @@ LIB_NAME = "libsarsolver_hip.so"
 LIB_PATH = PKG / LIB_NAME
 
 SOURCES = ["sar_kernels.hip", "sar_plan.hip", "conv_kernels.hip", "resample_kernels.hip",
-           "scene_kernels.hip"]
+           "scene_kernels.hip", "fft_kernels.hip"]
 
 # -ffp-contract=off: FP64 range arithmetic keeps the reference's roundings.
 # Packed FP32 instructions are disabled for all device code.
@@ -41,14 +41,23 @@ def build(verbose: bool = False) -> Path:
     build_dir = PKG / "_build"
     build_dir.mkdir(exist_ok=True)
     cc = hipcc()
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = build_dir / (Path(src).stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src), "-o", str(obj)]
+        cmds.append([cc, *HIPCC_FLAGS, "-I", str(ROOT / "include"), "-c", str(CSRC / src),
+                     "-o", str(obj)])
+        objs.append(str(obj))
+    # the sources compile independently: run them side by side
+    from concurrent.futures import ThreadPoolExecutor
+    workers = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(str(obj))
+
+    with ThreadPoolExecutor(workers) as pool:
+        list(pool.map(run, cmds))
     tmp = LIB_PATH.with_suffix(".so.tmp")
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", str(tmp),
            "-L/opt/rocm/lib", "-lrocfft", "-Wl,-rpath,/opt/rocm/lib"]

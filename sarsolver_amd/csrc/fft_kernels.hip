@@ -1,0 +1,463 @@
+// fft_kernels.hip -- fused spectral stages of the operators (gfx950).
+//
+// The operators move between an N-sample band (the phase-history row) and an
+// N_w = U N sample range profile (CPP:292, CPP:331-344).  Zero-padding the
+// band to N_w and running a length-N_w FFT reads and writes the padded row
+// several times.  With U an integer the transform splits into U polyphase
+// components, each an N-point DFT:
+//   adjoint  h[U m + r] = sum_k (B[k] e^{+2 pi i s_k r / N_w}) e^{+2 pi i k m / N}
+//   forward  X[k]       = sum_r e^{-2 pi i s_k r / N_w} sum_m h[U m + r] e^{-2 pi i k m / N}
+// (k the FFT-order band index, s_k its signed frequency).  One workgroup
+// takes one pulse row; G phases r are transformed at a time, one 256-thread
+// group each, with a Stockham FFT in LDS (radix-16 passes, a radix-2/4/8 pass
+// when N is not a power of 16) and coalesced G-phase loads / stores of the
+// profile row.  Band weights (conj(W) w_p on the way in, W w_p on the way
+// out), the centred / FFT order and the adjoint row tail are fused.
+#include <hip/hip_runtime.h>
+
+#include <cstdlib>
+
+#include "sar_device.h"
+#include "sar_launch.h"
+
+namespace sarhip {
+
+namespace {
+
+__device__ __forceinline__ float2 cmulf2(const float2 a, const float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 conjf2(const float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 addf2(const float2 a, const float2 b) {
+    return make_float2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ float2 subf2(const float2 a, const float2 b) {
+    return make_float2(a.x - b.x, a.y - b.y);
+}
+// i * a (INV) or -i * a
+template <bool INV>
+__device__ __forceinline__ float2 rot(const float2 a) {
+    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+}
+
+// LDS index with one pad slot per 16 (spreads the radix-16 writes over banks).
+__device__ __forceinline__ int phys(const int i) { return i + (i >> 4); }
+
+// In-register DFTs, out[q] = sum_r v[r] e^{sign 2 pi i r q / R}, sign = +1 for INV.
+template <bool INV>
+__device__ __forceinline__ void dft2(float2 &a, float2 &b) {
+    const float2 t = a;
+    a = addf2(t, b);
+    b = subf2(t, b);
+}
+template <bool INV>
+__device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d) {
+    const float2 s0 = addf2(a, c), d0 = subf2(a, c);
+    const float2 s1 = addf2(b, d), d1 = rot<INV>(subf2(b, d));
+    a = addf2(s0, s1);
+    c = subf2(s0, s1);
+    b = addf2(d0, d1);
+    d = subf2(d0, d1);
+}
+
+// e^{sign 2 pi i j / 16}, j = 0..9 (the products b c of the 4 x 4 split).
+__device__ __forceinline__ float2 w16(const int j, const bool inv) {
+    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, H = 0.70710678118654752f;
+    float2 w;
+    switch (j) {
+        case 0: w = make_float2(1.f, 0.f); break;
+        case 1: w = make_float2(C1, S1); break;
+        case 2: w = make_float2(H, H); break;
+        case 3: w = make_float2(S1, C1); break;
+        case 4: w = make_float2(0.f, 1.f); break;
+        case 6: w = make_float2(-H, H); break;
+        default: w = make_float2(-C1, -S1); break;  // j = 9
+    }
+    return inv ? w : conjf2(w);
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void dft(float2 (&v)[R]) {
+    if constexpr (R == 2) {
+        dft2<INV>(v[0], v[1]);
+    } else if constexpr (R == 4) {
+        dft4<INV>(v[0], v[1], v[2], v[3]);
+    } else if constexpr (R == 8) {
+        // r = 2a + b, q = c + 4d: DFT4 over a, twiddle e^{sign 2 pi i b c / 8}, DFT2 over b
+        dft4<INV>(v[0], v[2], v[4], v[6]);
+        dft4<INV>(v[1], v[3], v[5], v[7]);
+        constexpr float H = 0.70710678118654752f;
+        v[3] = cmulf2(v[3], INV ? make_float2(H, H) : make_float2(H, -H));
+        v[5] = rot<INV>(v[5]);
+        v[7] = cmulf2(v[7], INV ? make_float2(-H, H) : make_float2(-H, -H));
+        float2 o[8];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float2 a = v[2 * c], b = v[2 * c + 1];
+            dft2<INV>(a, b);
+            o[c] = a;
+            o[c + 4] = b;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = o[i];
+    } else {
+        static_assert(R == 16, "radix 2, 4, 8 or 16");
+        // r = 4a + b, q = c + 4d: DFT4 over a (per b), twiddle w16^{b c}, DFT4 over b (per c)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) dft4<INV>(v[b], v[b + 4], v[b + 8], v[b + 12]);
+        // after the first DFT4, v[b + 4 c] holds A_b[c]
+#pragma unroll
+        for (int b = 1; b < 4; ++b)
+#pragma unroll
+            for (int c = 1; c < 4; ++c) v[b + 4 * c] = cmulf2(v[b + 4 * c], w16(b * c, INV));
+        float2 o[16];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float2 a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
+            dft4<INV>(a0, a1, a2, a3);
+            o[c] = a0;
+            o[c + 4] = a1;
+            o[c + 8] = a2;
+            o[c + 12] = a3;
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = o[i];
+    }
+}
+
+// Twiddle tables in LDS, two levels each: e^{-2 pi i t / N} =
+// nlo[t & 63] nhi[t >> 6] (t < N) and e^{-2 pi i t / N_w} = wlo[t & 255]
+// whi[t >> 8] (t < N_w); every entry is rounded once from FP64, so a twiddle
+// carries about 1.5 ulp.
+struct Twiddles {
+    const float2 *nlo, *nhi, *wlo, *whi;
+    __device__ __forceinline__ float2 n(const int t) const { return cmulf2(nhi[t >> 6], nlo[t & 63]); }
+    __device__ __forceinline__ float2 w(const int t) const { return cmulf2(whi[t >> 8], wlo[t & 255]); }
+};
+// Host-side layout of the table array: [64 | N/64 | 256 | N_w/256] entries.
+template <int N>
+__device__ __forceinline__ Twiddles load_twiddles(float2 *dst, const float2 *__restrict__ src,
+                                                  int nw) {
+    const int total = 64 + N / 64 + 256 + nw / 256;
+    for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = src[i];
+    return Twiddles{dst, dst + 64, dst + 64 + N / 64, dst + 64 + N / 64 + 256};
+}
+
+// One Stockham pass of radix R with span Ns, in place on the group's LDS
+// buffer sh (all threads of the workgroup reach both barriers).
+template <int N, int R, int Ns, bool INV>
+__device__ __forceinline__ void stockham_pass(float2 *sh, const Twiddles &tw, int gt) {
+    constexpr int NB = N / R;
+    constexpr int PER = (NB + 255) / 256;
+    float2 v[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = gt + 256 * b;
+        if (j < NB)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = sh[phys(j + r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = gt + 256 * b;
+        if (j < NB) {
+            const int jm = j % Ns;
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 t = tw.n(jm * r * (N / (Ns * R)));
+                    v[b][r] = cmulf2(v[b][r], INV ? conjf2(t) : t);
+                }
+            }
+            dft<R, INV>(v[b]);
+            const int idx = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) sh[phys(idx + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+
+// All passes of an N-point transform on sh (natural order in and out):
+// radix 16 while a factor 16 is left, then the remaining 2, 4 or 8.
+template <int N, int Ns, bool INV>
+__device__ __forceinline__ void fft_passes(float2 *sh, const Twiddles &tw, int gt) {
+    if constexpr (Ns < N) {
+        constexpr int rest = N / Ns;
+        constexpr int R = rest >= 16 ? 16 : rest;
+        stockham_pass<N, R, Ns, INV>(sh, tw, gt);
+        fft_passes<N, Ns * R, INV>(sh, tw, gt);
+    }
+}
+template <int N, bool INV>
+__device__ __forceinline__ void fft_lds(float2 *sh, const Twiddles &tw, int gt) {
+    static_assert(N >= 64 && N <= 8192 && (N & (N - 1)) == 0, "N: power of two in [64, 8192]");
+    fft_passes<N, 1, INV>(sh, tw, gt);
+}
+
+__device__ __forceinline__ int signed_freq(const int k, const int N) { return k < N / 2 ? k : k - N; }
+
+// Q phases per batch share the LDS (Q buffers): two 256-thread groups (one
+// for Q = 1) take Q / 2 phases each in turn, then every m's Q consecutive
+// phases are one contiguous store / load of 8 Q bytes of the profile row.
+template <int Q>
+struct Batch {
+    static constexpr int G = Q >= 2 ? 2 : 1;  // thread groups
+    static constexpr int PPG = Q / G;         // phases per group
+    static constexpr int NT = 256 * G;        // threads
+};
+
+template <int Q>
+__device__ __forceinline__ void store_phases(float2 *dst, const float2 (&v)[Q]) {
+    if constexpr (Q == 1) {
+        dst[0] = v[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q += 2)
+            *(float4 *)(dst + q) = make_float4(v[q].x, v[q].y, v[q + 1].x, v[q + 1].y);
+    }
+}
+template <int Q>
+__device__ __forceinline__ void load_phases(const float2 *src, float2 (&v)[Q]) {
+    if constexpr (Q == 1) {
+        v[0] = src[0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q += 2) {
+            const float4 a = *(const float4 *)(src + q);
+            v[q] = make_float2(a.x, a.y);
+            v[q + 1] = make_float2(a.z, a.w);
+        }
+    }
+}
+
+// Adjoint: prof[p][n] (row stride ld) = sum_k B[k] e^{2 pi i s_k n / N_w},
+// B[k] = conj(W[k]) wfac[p] Y[p][src(k)]; bins [N_w, ld) repeat bins [0, ld - N_w).
+// B is formed once per row, in registers (k = gt + 256 i of each group).
+template <int N, int Q>
+__global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const float2 *__restrict__ Y,
+                                                              const float2 *__restrict__ Wconj,
+                                                              const float2 *__restrict__ wfac,
+                                                              int U, int ld, int centred,
+                                                              const float2 *__restrict__ tables,
+                                                              float2 *__restrict__ prof) {
+    using Bt = Batch<Q>;
+    extern __shared__ float2 lds[];
+    constexpr int SZ = N + N / 16;
+    constexpr int NI = (N + 255) / 256;
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x, g = tid >> 8, gt = tid & 255;
+    const int nw = U * N;
+    const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
+    const float2 wp = wfac[p];
+    const float2 *y = Y + (size_t)p * N;
+    float2 *row = prof + (size_t)p * ld;
+    float2 B[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int k = gt + 256 * i;
+        if (k < N) {
+            const int src = centred ? (k + N / 2) % N : k;
+            B[i] = cmulf2(cmulf2(Wconj[k], y[src]), wp);
+        }
+    }
+    for (int r0 = 0; r0 < U; r0 += Q) {
+        __syncthreads();  // tables loaded / previous batch written out
+#pragma unroll 1
+        for (int pp = 0; pp < Bt::PPG; ++pp) {
+            const int buf = g + Bt::G * pp, rp = r0 + buf;
+            float2 *sh = lds + buf * SZ;
+            if constexpr (N >= 4096) {
+                // the first radix-16 pass straight from registers: butterfly
+                // j = gt + 256 b takes k = j + (N / 16) r, held in B[b + (N / 4096) r]
+                constexpr int PB = N / 4096;
+#pragma unroll
+                for (int b = 0; b < PB; ++b) {
+                    float2 v[16];
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int k = gt + 256 * (b + PB * r);
+                        int t = (signed_freq(k, N) * rp) % nw;
+                        t = t < 0 ? t + nw : t;
+                        v[r] = cmulf2(B[b + PB * r], conjf2(tw.w(t)));
+                    }
+                    dft<16, true>(v);
+                    const int j = gt + 256 * b;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) sh[phys(16 * j + r)] = v[r];
+                }
+                __syncthreads();
+                fft_passes<N, 16, true>(sh, tw, gt);
+            } else {
+#pragma unroll
+                for (int i = 0; i < NI; ++i) {
+                    const int k = gt + 256 * i;
+                    if (k < N) {
+                        int t = (signed_freq(k, N) * rp) % nw;
+                        t = t < 0 ? t + nw : t;
+                        sh[phys(k)] = cmulf2(B[i], conjf2(tw.w(t)));
+                    }
+                }
+                __syncthreads();
+                fft_lds<N, true>(sh, tw, gt);
+            }
+        }
+        // Q consecutive phases of each m are contiguous in the row
+#pragma unroll 2
+        for (int m = tid; m < N; m += Bt::NT) {
+            const size_t n = (size_t)U * m + r0;
+            float2 v[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) v[q] = lds[q * SZ + phys(m)];
+            store_phases<Q>(row + n, v);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (n + q < (size_t)(ld - nw)) row[nw + n + q] = v[q];
+        }
+    }
+}
+
+// Forward: Y[p][dst(k)] = W[k] wfac[p] sum_n prof[p][n] e^{-2 pi i s_k n / N_w}
+// (row stride nw = U N), k over the N band bins.
+template <int N, int Q>
+__global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__restrict__ prof,
+                                                                const float2 *__restrict__ W,
+                                                                const float2 *__restrict__ wfac,
+                                                                int U, int centred,
+                                                                const float2 *__restrict__ tables,
+                                                                float2 *__restrict__ Y) {
+    using Bt = Batch<Q>;
+    extern __shared__ float2 lds[];
+    constexpr int SZ = N + N / 16;
+    constexpr int PER = (N + Bt::NT - 1) / Bt::NT;
+    const int p = blockIdx.x;
+    const int tid = threadIdx.x, g = tid >> 8, gt = tid & 255;
+    const int nw = U * N;
+    const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
+    const float2 *row = prof + (size_t)p * nw;
+    float2 acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int r0 = 0; r0 < U; r0 += Q) {
+#pragma unroll 2
+        for (int m = tid; m < N; m += Bt::NT) {
+            float2 v[Q];
+            load_phases<Q>(row + (size_t)U * m + r0, v);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) lds[q * SZ + phys(m)] = v[q];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int pp = 0; pp < Bt::PPG; ++pp) fft_lds<N, false>(lds + (g + Bt::G * pp) * SZ, tw, gt);
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int k = tid + Bt::NT * q;
+            if (k >= N) break;
+            const int s = signed_freq(k, N);
+#pragma unroll
+            for (int qq = 0; qq < Q; ++qq) {
+                int t = (s * (r0 + qq)) % nw;
+                t = t < 0 ? t + nw : t;
+                acc[q] = addf2(acc[q], cmulf2(lds[qq * SZ + phys(k)], tw.w(t)));
+            }
+        }
+        __syncthreads();
+    }
+    const float2 wp = wfac[p];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int k = tid + Bt::NT * q;
+        if (k >= N) break;
+        const int dst = centred ? (k + N / 2) % N : k;
+        Y[(size_t)p * N + dst] = cmulf2(cmulf2(W[k], acc[q]), wp);
+    }
+}
+
+// LDS: Q phase buffers, then the twiddle tables.
+template <int N, int Q>
+size_t lds_bytes(int U) {
+    return sizeof(float2) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + (size_t)U * N / 256);
+}
+
+template <int N, int Q>
+hipError_t up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int U, int ld, int P,
+              int centred, const float2 *tables, float2 *prof, hipStream_t s) {
+    const size_t lds = lds_bytes<N, Q>(U);
+    hipError_t e = hipFuncSetAttribute((const void *)fft_up_kernel<N, Q>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    fft_up_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(Y, Wconj, wfac, U, ld, centred, tables, prof);
+    return hipGetLastError();
+}
+
+template <int N, int Q>
+hipError_t down(const float2 *prof, const float2 *W, const float2 *wfac, int U, int P, int centred,
+                const float2 *tables, float2 *Y, hipStream_t s) {
+    const size_t lds = lds_bytes<N, Q>(U);
+    hipError_t e = hipFuncSetAttribute((const void *)fft_down_kernel<N, Q>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    fft_down_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(prof, W, wfac, U, centred, tables, Y);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+// Phases per batch: the largest Q of 8, 4, 2, 1 that divides U, whose Q
+// buffers and the tables fit the LDS, and not above a cap: 8 for the adjoint
+// (contiguous stores matter most), 2 for the forward (measured at N = 4096,
+// U = 8: 0.39 ms against 0.41 ms with 4); SARHIP_FFT_PHASES overrides both.
+int fused_fft_phases(int N, int U, bool up) {
+    const size_t per = sizeof(float2) * (size_t)(N + N / 16);
+    const size_t tab = sizeof(float2) * (64 + N / 64 + 256 + (size_t)U * N / 256);
+    int cap = up ? 8 : 2;
+    if (const char *e = std::getenv("SARHIP_FFT_PHASES")) cap = std::atoi(e);
+    for (int Q : {8, 4, 2}) {
+        if (Q <= cap && U % Q == 0 && Q * per + tab <= 160 * 1024) return Q;
+    }
+    return 1;
+}
+
+// N a power of two in [64, 8192], N_w = U N a multiple of 256 up to 2^17
+// (the two-level twiddle tables stay small).
+bool fused_fft_supported(int N, int U) {
+    const long long nw = (long long)U * N;
+    return N >= 64 && N <= 8192 && (N & (N - 1)) == 0 && U >= 1 && nw % 256 == 0 &&
+           nw <= (1 << 17);
+}
+
+#define SARHIP_FFT_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+
+hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N, int U,
+                         int ld, int P, int centred, const float2 *tables, float2 *prof,
+                         hipStream_t s) {
+    const int Q = fused_fft_phases(N, U, true);
+#define UP_CASE(n)                                                                       \
+    if (N == n) {                                                                        \
+        if (Q == 8) return up<n, 8>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        if (Q == 4) return up<n, 4>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        if (Q == 2) return up<n, 2>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        return up<n, 1>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s);             \
+    }
+    SARHIP_FFT_SIZES(UP_CASE)
+#undef UP_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_fft_down(const float2 *prof, const float2 *W, const float2 *wfac, int N, int U,
+                           int P, int centred, const float2 *tables, float2 *Y, hipStream_t s) {
+    const int Q = fused_fft_phases(N, U, false);
+#define DOWN_CASE(n)                                                                     \
+    if (N == n) {                                                                        \
+        if (Q == 8) return down<n, 8>(prof, W, wfac, U, P, centred, tables, Y, s);       \
+        if (Q == 4) return down<n, 4>(prof, W, wfac, U, P, centred, tables, Y, s);       \
+        if (Q == 2) return down<n, 2>(prof, W, wfac, U, P, centred, tables, Y, s);       \
+        return down<n, 1>(prof, W, wfac, U, P, centred, tables, Y, s);                   \
+    }
+    SARHIP_FFT_SIZES(DOWN_CASE)
+#undef DOWN_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace sarhip

@@ -931,6 +931,10 @@ hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s) {
         else if (k.interp == kInterpNearest) bp_dispatch_prec<kInterpNearest, false>(k, s);
         else bp_dispatch_prec<kInterpNearestRef, false>(k, s);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_bp_reduce(const KernelParams &k, hipStream_t s) {
     if (k.bp_splits > 1)
         bp_reduce_kernel<<<(k.M + 255) / 256, 256, 0, s>>>(k.bp_partial, k.M, k.bp_splits, k.perm,
                                                            k.out);
@@ -963,11 +967,15 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
 }
 
 // 64 pulses per workgroup run with 512 threads (8 pixel streams), fewer with 256.
-template <int I, bool MO>
-static void fp_dispatch_threads(const KernelParams &k, hipStream_t s) {
+hipError_t launch_region_l1(const KernelParams &k, hipStream_t s) {
     region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(const_cast<float2 *>(k.amps_perm), k.M,
                                                         k.region_tiles * kTilePixels, k.win > 0,
                                                         k.region_l1);
+    return hipGetLastError();
+}
+
+template <int I, bool MO>
+static void fp_dispatch_threads(const KernelParams &k, hipStream_t s) {
     if (k.fwd_pulses == 64) fp_dispatch_prec<I, MO, 512>(k, s);
     else fp_dispatch_prec<I, MO, kThreads>(k, s);
 }

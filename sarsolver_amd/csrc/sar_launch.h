@@ -94,6 +94,8 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
                                 const int *rows, int Pv, double by_dx, double kc_rev, bool mono,
                                 void *out, hipStream_t s);
 hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
+hipError_t launch_bp_reduce(const KernelParams &k, hipStream_t s);  // pulse-split images
+hipError_t launch_region_l1(const KernelParams &k, hipStream_t s);  // before launch_fp_bin
 hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
                            double nwd, double *out, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)
@@ -103,6 +105,16 @@ hipError_t launch_rs_window(const float2 *prof, int N, int N2, const float *win,
                             int K, const int *starts, int P2, float2 *out, hipStream_t s);
 hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, const float *sf,
                                 float2 *out, hipStream_t s);
+// fused spectral stages (fft_kernels.hip): N-sample band <-> U N-sample rows
+bool fused_fft_supported(int N, int U);
+int fused_fft_phases(int N, int U, bool up);
+// tables: [64 | N/64 | 256 | U N/256] two-level twiddle tables,
+// e^{-2 pi i q / N}, e^{-2 pi i 64 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
+hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N, int U,
+                         int ld, int P, int centred, const float2 *tables, float2 *prof,
+                         hipStream_t s);
+hipError_t launch_fft_down(const float2 *prof, const float2 *W, const float2 *wfac, int N, int U,
+                           int P, int centred, const float2 *tables, float2 *Y, hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

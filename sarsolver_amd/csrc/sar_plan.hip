@@ -126,6 +126,8 @@ struct sarhip_plan_s {
     double4 *d_tile_c = nullptr, *d_region_c = nullptr, *d_region_h = nullptr;
     PixRec *d_pixrec = nullptr;
     void *d_series = nullptr;
+    int fused_U = 0;                                // > 0: fused spectral kernels, N_w = U N
+    float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
@@ -162,6 +164,7 @@ struct sarhip_plan_s {
         dev_free(d_pixrec);
         dev_free(d_series);
         dev_free(d_bp_cu);
+        dev_free(d_fft_tables);
         dev_free(d_W);
         dev_free(d_Wconj);
         dev_free(d_wfac_fwd);
@@ -240,14 +243,17 @@ int run_fft(sarhip_plan_t pl, rocfft_plan fp, void *buf, hipStream_t s) {
 }
 
 constexpr size_t kMaxTimedCalls = 4096;
+// Events per timed call: stage boundaries 0..3, then 4 / 5 around the main
+// kernel's launch (scatter or gather alone).
+constexpr size_t kEventsPerCall = 6;
 
 // Reserve the next event set of `which` (0 forward, 1 adjoint) for this call.
 int begin_timing(sarhip_plan_t pl, int which) {
     pl->ev = nullptr;
     if (!pl->timing || pl->ev_used[which] >= kMaxTimedCalls) return SARHIP_OK;
     auto &pool = pl->ev_pool[which];
-    const size_t base = 4 * pl->ev_used[which];
-    while (pool.size() < base + 4) {
+    const size_t base = kEventsPerCall * pl->ev_used[which];
+    while (pool.size() < base + kEventsPerCall) {
         hipEvent_t e = nullptr;
         HIP_TRY(hipEventCreate(&e));
         pool.push_back(e);
@@ -810,6 +816,36 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         }
     }
 
+    // Spectral stages: the fused polyphase kernels when N_w = U N with an
+    // integer U and a supported N (twiddle tables from FP64), else rocFFT.
+    {
+        const size_t U = nw / N;
+        pl->fused_U = 0;
+        if (U * N == nw && fused_fft_supported((int)N, (int)U) && env_int("SARHIP_FUSED_FFT", 1)) {
+            // two-level twiddle tables, each entry rounded once from FP64
+            std::vector<float2> tab;
+            auto put = [&](size_t count, double step, size_t len) {
+                for (size_t q = 0; q < count; ++q) {
+                    const double a = -2.0 * M_PI * (double)(q * step) / (double)len;
+                    tab.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+                }
+            };
+            put(64, 1, N);
+            put(N / 64, 64, N);
+            put(256, 1, nw);
+            put(nw / 256, 256, nw);
+            if ((rc = upload((void **)&pl->d_fft_tables, tab.data(), sizeof(float2) * tab.size())))
+                return rc;
+            pl->fused_U = (int)U;
+        }
+    }
+    if (pl->fused_U > 0) {
+        HIP_TRY(hipDeviceSynchronize());
+        owner.p = nullptr;
+        *out = pl;
+        return SARHIP_OK;
+    }
+
     // rocFFT: batched in-place complex64 transforms of length N_w.
     const size_t len = nw;
     FFT_TRY(rocfft_plan_create(&pl->fft_fwd, rocfft_placement_inplace,
@@ -970,8 +1006,10 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *nw, double *by_dx, double *kc) 
 
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
                        int *fwd_window_bins, int *fwd_pulses_per_group, double *series_err_bins,
-                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins) {
+                       int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins,
+                       int *fused_fft_ratio) {
     if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
+    if (fused_fft_ratio) *fused_fft_ratio = plan->fused_U;
     if (bp_series_order) *bp_series_order = plan->series_order;
     if (bp_series_err_bins) *bp_series_err_bins = plan->bp_series_err_bins;
     if (fwd_region_tiles) *fwd_region_tiles = plan->region_tiles;
@@ -997,7 +1035,7 @@ int sarhip_plan_timing(sarhip_plan_t plan, int which, float *ms4, int *ncalls) {
     double acc[3] = {0, 0, 0};
     const size_t n = plan->ev_used[which];
     const hipEvent_t *e = plan->ev_pool[which].data();
-    for (size_t c = 0; c < n; ++c, e += 4) {
+    for (size_t c = 0; c < n; ++c, e += kEventsPerCall) {
         HIP_TRY(hipEventSynchronize(e[3]));
         for (int k = 0; k < 3; ++k) {
             float ms = 0.f;
@@ -1017,6 +1055,23 @@ int sarhip_plan_timing(sarhip_plan_t plan, int which, float *ms4, int *ncalls) {
     return SARHIP_OK;
 }
 
+int sarhip_plan_kernel_timing(sarhip_plan_t plan, int which, float *ms, int *ncalls) {
+    if (!plan || !ms || (which != 0 && which != 1)) return fail(SARHIP_EINVAL, "bad argument");
+    DeviceGuard g(plan->device);
+    double acc = 0.0;
+    const size_t n = plan->ev_used[which];
+    const hipEvent_t *e = plan->ev_pool[which].data();
+    for (size_t c = 0; c < n; ++c, e += kEventsPerCall) {
+        HIP_TRY(hipEventSynchronize(e[3]));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, e[4], e[5]));
+        acc += t;
+    }
+    *ms = (float)(n ? acc / (double)n : 0.0);
+    if (ncalls) *ncalls = (int)n;
+    return SARHIP_OK;
+}
+
 int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int order,
                    void *stream) {
     if (!pl || !amps || !phase_history) return fail(SARHIP_EINVAL, "NULL argument");
@@ -1032,10 +1087,25 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
         HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm,
                                    s));
         const KernelParams k = pl->params();
+        HIP_TRY(launch_region_l1(k, s));
+        if ((rc = record(pl, 4, s))) return rc;
         HIP_TRY(launch_fp_bin(k, s));
+        if ((rc = record(pl, 5, s))) return rc;
         if (pl->win > 0) HIP_TRY(launch_fp_reduce(k, pl->reduce_chunk, s));
+    } else {
+        if ((rc = record(pl, 4, s))) return rc;
+        if ((rc = record(pl, 5, s))) return rc;
     }
     if ((rc = record(pl, 1, s))) return rc;
+    if (pl->fused_U > 0) {
+        // FFT and truncation in one pass over the profile rows
+        HIP_TRY(launch_fft_down(pl->d_prof, pl->d_W, pl->d_wfac_fwd, (int)pl->N, pl->fused_U,
+                                (int)pl->P, order == SARHIP_ORDER_CENTRED, pl->d_fft_tables,
+                                (float2 *)phase_history, s));
+        if ((rc = record(pl, 2, s))) return rc;
+        if ((rc = record(pl, 3, s))) return rc;
+        return SARHIP_OK;
+    }
     if ((rc = run_fft(pl, pl->fft_fwd, pl->d_prof, s))) return rc;
     if ((rc = record(pl, 2, s))) return rc;
     HIP_TRY(launch_truncate(pl->d_prof, pl->d_W, pl->d_wfac_fwd, (int)pl->N, (int)pl->nw,
@@ -1054,16 +1124,27 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     int rc;
     if ((rc = begin_timing(pl, 1))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
-    HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
-                              (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,
-                              order == SARHIP_ORDER_CENTRED, pl->d_prof, s));
-    if ((rc = record(pl, 1, s))) return rc;
-    if ((rc = run_fft(pl, pl->fft_bwd, pl->d_prof, s))) return rc;
-    HIP_TRY(launch_wrap_pad(pl->d_prof, (int)pl->nw, (int)pl->ld, (int)pl->P, s));
+    if (pl->fused_U > 0) {
+        // band weights, zero padding, inverse FFT and row tails in one pass
+        if ((rc = record(pl, 1, s))) return rc;
+        HIP_TRY(launch_fft_up((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
+                              (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
+                              order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof, s));
+    } else {
+        HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
+                                  (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,
+                                  order == SARHIP_ORDER_CENTRED, pl->d_prof, s));
+        if ((rc = record(pl, 1, s))) return rc;
+        if ((rc = run_fft(pl, pl->fft_bwd, pl->d_prof, s))) return rc;
+        HIP_TRY(launch_wrap_pad(pl->d_prof, (int)pl->nw, (int)pl->ld, (int)pl->P, s));
+    }
     if ((rc = record(pl, 2, s))) return rc;
     KernelParams k = pl->params();
     k.out = (float2 *)amps;
+    if ((rc = record(pl, 4, s))) return rc;
     HIP_TRY(launch_bp_gather(k, s));
+    if ((rc = record(pl, 5, s))) return rc;
+    HIP_TRY(launch_bp_reduce(k, s));
     if ((rc = record(pl, 3, s))) return rc;
     return SARHIP_OK;
 }
