@@ -484,15 +484,11 @@ __device__ __forceinline__ float2 load_uniform(const float2 *p, size_t i) {
     return make_float2(__uint_as_float((unsigned)__double_as_longlong(v)),
                        __uint_as_float((unsigned)(__double_as_longlong(v) >> 32)));
 }
-// (re, im) -> im * 2^32 + re as a 64-bit two's-complement word
-__device__ __forceinline__ unsigned long long pack_fix(const int re, const int im) {
-    return (unsigned long long)(((long long)im << 32) + (long long)re);
-}
-// inverse of a sum of pack_fix words whose component sums lie in (-2^31, 2^31)
-__device__ __forceinline__ int2 unpack_fix(const unsigned long long v) {
-    const int re = (int)(unsigned)(v & 0xffffffffull);
-    const int im = (int)(((long long)v - (long long)re) >> 32);
-    return make_int2(re, im);
+// Fixed-point window bin: two 32-bit words (re, im), each accumulated by its
+// own 32-bit LDS add (no carry between the components).
+__device__ __forceinline__ void add_fix(unsigned int *bin, const int re, const int im) {
+    atomicAdd(bin, (unsigned)re);
+    atomicAdd(bin + 1, (unsigned)im);
 }
 
 // Per-region L1 norm sum(|Re a| + |Im a|) of the permuted amplitudes (FP32,
@@ -530,12 +526,14 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict_
 
 template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT, int SORD = 5>
 __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
-    // Fixed-point windows [G][wst], one 64-bit word per bin holding
-    // im * 2^32 + re: a single 64-bit LDS add accumulates both components
-    // (the low word's carries land in the high word, and both parts are
-    // recovered exactly because each stays below 2^31 in magnitude).  The
-    // per-pulse stride wst is odd, spreading a wave's lanes (distinct pulses,
-    // nearby bins) over the LDS banks.
+    // Fixed-point windows [G][wst] of bins, a bin being two 32-bit words
+    // (re, im) accumulated by 32-bit LDS adds: the four adds of a linear
+    // contribution share one address (immediate offsets 0, 4, 8, 12), and
+    // each component sum stays below 2^31 in magnitude.  (One 64-bit add of
+    // im * 2^32 + re needs a sign extension and a 64-bit shift-add per
+    // contribution, five VALU instructions more.)  The per-pulse stride wst is
+    // odd, spreading a wave's lanes (distinct pulses, nearby bins) over the
+    // LDS banks.
     extern __shared__ unsigned long long wsum[];
     const int G = NT == 512 ? 64 : k.fwd_pulses, R = k.region_tiles;
     // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so the
@@ -596,7 +594,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const int wst = k.win_stride;
     for (int e = tid; e < G * wst; e += NT) wsum[e] = 0ull;
     __syncthreads();
-    unsigned long long *ws = wsum + q * wst;
+    unsigned int *ws = (unsigned int *)(wsum + q * wst);
     float2 *row = k.prof + (size_t)p * k.nw;
     if (!(fixed_ok && scale == 0.f)) {  // skip all-zero regions
         for (int tile = t0; tile < t1; ++tile) {
@@ -666,8 +664,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         if (qok) {
                             const int tr = fix_rn(cr), ti = fix_rn(ci);
                             const int r1 = fix_rn(fr * cr), i1 = fix_rn(fr * ci);
-                            atomicAdd(&ws[l], pack_fix(tr - r1, ti - i1));
-                            atomicAdd(&ws[l + 1], pack_fix(r1, i1));
+                            add_fix(ws + 2 * l, tr - r1, ti - i1);
+                            add_fix(ws + 2 * l + 2, r1, i1);
                         }
                     } else if (qok) {
                         const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
@@ -680,7 +678,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     }
                 } else {
                     if (fixed_ok && (unsigned)l < (unsigned)win) {
-                        if (qok) atomicAdd(&ws[l], pack_fix(fix_rn(cr), fix_rn(ci)));
+                        if (qok) add_fix(ws + 2 * l, fix_rn(cr), fix_rn(ci));
                     } else if (qok) {
                         const int nb = bin_index(k, fl, 1);
                         atomicAdd(&row[nb].x, cr * unscale);
@@ -699,7 +697,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
         float mx = 0.f;
         float2 *dst = k.partial + ((size_t)(j0 + qq) * NR + region) * win;
         for (int l = lane; l < win; l += 64) {
-            const int2 v = unpack_fix(wsum[qq * wst + l]);
+            const unsigned long long w = wsum[qq * wst + l];
+            const int2 v = make_int2((int)(unsigned)(w & 0xffffffffull), (int)(unsigned)(w >> 32));
             const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
             dst[l] = f;
             mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
