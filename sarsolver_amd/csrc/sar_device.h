@@ -151,4 +151,12 @@ struct PixRec {
 };
 static_assert(sizeof(PixRec) == 64, "PixRec must be 64 bytes");
 
+// Pixel record of the planar linear scatter (32 bytes, tile order): offsets
+// from the tile centre, |offset|^2 and the pixel's (scaled) amplitude.
+struct ScatRec {
+    double dx, dy, d2;
+    float2 a;
+};
+static_assert(sizeof(ScatRec) == 32, "ScatRec must be 32 bytes");
+
 }  // namespace sarhip

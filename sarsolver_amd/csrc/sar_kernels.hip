@@ -484,6 +484,23 @@ __device__ __forceinline__ float2 load_uniform(const float2 *p, size_t i) {
     return make_float2(__uint_as_float((unsigned)__double_as_longlong(v)),
                        __uint_as_float((unsigned)(__double_as_longlong(v) >> 32)));
 }
+// Records i and i + 1 at a wave-uniform index: one 64-byte scalar load.
+__device__ __forceinline__ void load_srec2(const ScatRec *p, size_t i, ScatRec &a, ScatRec &b) {
+    const const_f64_ptr d = (const_f64_ptr)(p + i);
+    double v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = d[e];
+    a.dx = v[0];
+    a.dy = v[1];
+    a.d2 = v[2];
+    long long w = __double_as_longlong(v[3]);
+    a.a = make_float2(__uint_as_float((unsigned)w), __uint_as_float((unsigned)(w >> 32)));
+    b.dx = v[4];
+    b.dy = v[5];
+    b.d2 = v[6];
+    w = __double_as_longlong(v[7]);
+    b.a = make_float2(__uint_as_float((unsigned)w), __uint_as_float((unsigned)(w >> 32)));
+}
 // Fixed-point window bin: two 32-bit words (re, im), each accumulated by its
 // own 32-bit LDS add (no carry between the components).
 __device__ __forceinline__ void add_fix(unsigned int *bin, const int re, const int im) {
@@ -498,7 +515,9 @@ __device__ __forceinline__ void add_fix(unsigned int *bin, const int re, const i
 __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict__ amps,
                                                              int M, int region_pixels,
                                                              int windows,
-                                                             float *__restrict__ out) {
+                                                             float *__restrict__ out,
+                                                             const PixRec *__restrict__ pixrec,
+                                                             ScatRec *__restrict__ srec) {
     __shared__ float red[kThreads];
     const int base = blockIdx.x * region_pixels;
     const int n = min(region_pixels, M - base);
@@ -515,11 +534,26 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict_
     }
     const float tot = red[0];
     if (threadIdx.x == 0) out[blockIdx.x] = tot;
-    if (windows && isfinite(tot) && tot > 0.f) {
-        const float sc = fixed_scale(tot);
+    const bool scale = windows && isfinite(tot) && tot > 0.f;
+    const float sc = scale ? fixed_scale(tot) : 1.f;
+    if (scale || srec) {
         for (int i = threadIdx.x; i < n; i += kThreads) {
-            const float2 a = amps[base + i];
-            amps[base + i] = make_float2(a.x * sc, a.y * sc);
+            float2 a = amps[base + i];
+            if (scale) {
+                a = make_float2(a.x * sc, a.y * sc);
+                amps[base + i] = a;
+            }
+            if (srec) {
+                // planar scatter records: offsets from the tile centre and the
+                // (scaled) amplitude, zero for invalid pixels
+                const PixRec &q = pixrec[base + i];
+                ScatRec r;
+                r.dx = q.dx;
+                r.dy = q.dy;
+                r.d2 = q.d2;
+                r.a = q.ok != 0.0 ? a : make_float2(0.f, 0.f);
+                srec[base + i] = r;
+            }
         }
     }
 }
@@ -568,6 +602,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const float unscale = scale > 0.f ? inv_scale : 1.0f;
     // linear series / hilo: u is evaluated relative to the window base
     constexpr bool kRel = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
+    constexpr bool kCompact = kRel && PLANAR && NT == 512;
 
     // Window base: for a leg with phase centre T, unit vector e from the region
     // centre Xc to T and a pixel X = Xc + d in the region's box of half-extents h,
@@ -603,42 +638,26 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             if constexpr (kRel) d.s.cu -= wb;
             const int tb = tile * kTilePixels;
             const int tn = min(kTilePixels, k.M - tb);
-            // NT == 512: the next pixel's record is requested one iteration ahead
+            // NT == 512: the next pixel's record is requested one iteration ahead.
+            // kCompact: each wave walks a contiguous block of the tile's
+            // pixels through the 32-byte records (one scalar load per pixel,
+            // two records per cache line); otherwise the streams interleave.
             PixRec r_next;
             float2 a_next;
-            if constexpr (NT == 512) {
+            const int per = (tn + nstreams - 1) / nstreams;
+            const int ibeg = kCompact ? stream * per : stream;
+            const int iend = kCompact ? min(tn, ibeg + per) : tn;
+            const int istep = kCompact ? 1 : nstreams;
+            if constexpr (kCompact) {
+                // (loaded in pairs below)
+            } else if constexpr (NT == 512) {
                 if (stream < tn) {
                     r_next = load_uniform(k.pixrec, (size_t)(tb + stream));
                     a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + stream));
                 }
             }
-            for (int i = stream; i < tn; i += nstreams) {
-                PixRec r;
-                float2 a;
-                if constexpr (NT == 512) {
-                    // wave-uniform index: scalar loads through the constant address space
-                    r = r_next;
-                    a = a_next;
-                    if (i + nstreams < tn) {
-                        r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
-                        a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + i + nstreams));
-                    }
-                    // ok is exactly 0.0 or 1.0: an integer test keeps it on the scalar unit
-                    if (__double_as_longlong(r.ok) == 0) continue;
-                } else {
-                    r = k.pixrec[tb + i];
-                    if (r.ok == 0.0) continue;
-                    a = k.amps_perm[tb + i];
-                }
-                Pix px;
-                px.x = r.x;
-                px.y = r.y;
-                px.z = r.z;
-                px.dx = r.dx;
-                px.dy = r.dy;
-                px.dz = r.dz;
-                px.d2 = r.d2;
-                px.ok = true;
+            // one pixel's contribution to this lane's pulse window
+            auto contribute = [&](const Pix &px, const float2 a) {
                 double u, th;
                 eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
                 float sn, cs;
@@ -684,6 +703,72 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         atomicAdd(&row[nb].x, cr * unscale);
                         atomicAdd(&row[nb].y, ci * unscale);
                     }
+                }
+            };
+            if constexpr (kCompact) {
+                // records in groups of four (two 64-byte scalar loads), the
+                // next group requested while this one is processed: one wait
+                // per four pixels; invalid pixels carry a zero amplitude and
+                // a zero offset (the record array has four spare records at
+                // its end, so a group never reads past it)
+                constexpr int KB = 4;
+                ScatRec c[KB];
+                if (ibeg < iend) {
+                    load_srec2(k.srec, (size_t)(tb + ibeg), c[0], c[1]);
+                    load_srec2(k.srec, (size_t)(tb + ibeg + 2), c[2], c[3]);
+                }
+                for (int i = ibeg; i < iend; i += KB) {
+                    ScatRec cur[KB];
+#pragma unroll
+                    for (int e = 0; e < KB; ++e) cur[e] = c[e];
+                    if (i + KB < iend) {
+                        load_srec2(k.srec, (size_t)(tb + i + KB), c[0], c[1]);
+                        load_srec2(k.srec, (size_t)(tb + i + KB + 2), c[2], c[3]);
+                    }
+                    Pix px;
+                    px.x = px.y = px.z = 0.0;  // unused on this path
+                    px.dz = 0.0;
+                    px.ok = true;
+#pragma unroll
+                    for (int e = 0; e < KB; ++e) {
+                        if (i + e < iend) {
+                            px.dx = cur[e].dx;
+                            px.dy = cur[e].dy;
+                            px.d2 = cur[e].d2;
+                            contribute(px, cur[e].a);
+                        }
+                    }
+                }
+            } else {
+                for (int i = ibeg; i < iend; i += istep) {
+                    PixRec r;
+                    float2 a;
+                    if constexpr (NT == 512) {
+                        // wave-uniform index: scalar loads through the constant address space
+                        r = r_next;
+                        a = a_next;
+                        if (i + nstreams < tn) {
+                            r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
+                            a_next = load_uniform((const float2 *)k.amps_perm,
+                                                  (size_t)(tb + i + nstreams));
+                        }
+                        // ok is exactly 0.0 or 1.0: an integer test keeps it on the scalar unit
+                        if (__double_as_longlong(r.ok) == 0) continue;
+                    } else {
+                        r = k.pixrec[tb + i];
+                        if (r.ok == 0.0) continue;
+                        a = k.amps_perm[tb + i];
+                    }
+                    Pix px;
+                    px.x = r.x;
+                    px.y = r.y;
+                    px.z = r.z;
+                    px.dx = r.dx;
+                    px.dy = r.dy;
+                    px.dz = r.dz;
+                    px.d2 = r.d2;
+                    px.ok = true;
+                    contribute(px, a);
                 }
             }
         }
@@ -969,7 +1054,8 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
 hipError_t launch_region_l1(const KernelParams &k, hipStream_t s) {
     region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(const_cast<float2 *>(k.amps_perm), k.M,
                                                         k.region_tiles * kTilePixels, k.win > 0,
-                                                        k.region_l1);
+                                                        k.region_l1, k.pixrec,
+                                                        const_cast<ScatRec *>(k.srec));
     return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ struct KernelParams {
     int num_tiles;               // ceil(M / kTilePixels)
     const double4 *tile_c;       // per tile: series centre (multiple of 2^-10 m), radius
     const PixRec *pixrec;        // [M] per-pixel records (scatter kernels)
+    const ScatRec *srec;         // [M] planar scatter records, rewritten per forward call (or NULL)
     // pulses
     const PulseConst *pc;  // [P]
     const int *rows;       // valid pulse rows [Pv]

@@ -129,6 +129,7 @@ struct sarhip_plan_s {
     int fused_U = 0;                                // > 0: fused spectral kernels, N_w = U N
     float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
+    ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
     float2 *d_amps_perm = nullptr;
@@ -164,6 +165,7 @@ struct sarhip_plan_s {
         dev_free(d_pixrec);
         dev_free(d_series);
         dev_free(d_bp_cu);
+        dev_free(d_srec);
         dev_free(d_fft_tables);
         dev_free(d_W);
         dev_free(d_Wconj);
@@ -195,6 +197,7 @@ struct sarhip_plan_s {
         k.num_tiles = num_tiles;
         k.tile_c = d_tile_c;
         k.pixrec = d_pixrec;
+        k.srec = d_srec;
         k.pc = d_pc;
         k.rows = d_rows;
         k.Pv = Pv;
@@ -789,6 +792,11 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
+    // The planar 64-pulse linear scatter streams 32-byte records that the
+    // region-norm kernel writes on every forward call.
+    if (pl->planar && interp == SARHIP_INTERP_LINEAR && pl->fwd_pulses == 64 &&
+        (pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO))
+        HIP_TRY(hipMalloc((void **)&pl->d_srec, sizeof(ScatRec) * (M + 4)));  // + group overrun
     // Backprojection: split each tile's pulses over several workgroups until
     // the grid has about 8 workgroups per CU (occupancy), keeping at least 128
     // pulses per split; partial images are summed in FP64 in split order.
