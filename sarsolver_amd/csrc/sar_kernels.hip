@@ -335,40 +335,68 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
         ar[m] = ai[m] = 0.f;
         acc[2 * m][tid] = acc[2 * m + 1][tid] = 0.0;
     }
+    constexpr bool kShift = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
+    // The next pulse's row index and constants are requested one iteration
+    // ahead (scalar loads: a wait at the top of the loop would otherwise
+    // expose their latency on every pulse).  With kShift, u of the tile
+    // centre comes moved by a multiple of nw (bp_shift_kernel, once per
+    // plan) so that every pixel of the tile lands in [2, nw + 2 shift_lo - 2):
+    // bins are unchanged modulo nw and the padded row tail makes bins
+    // n + 1 <= ld - 1 valid without a wrap.
+    int p_next = 0;
+    PulseData<MONO> d_next;
+    if (j_begin < j_end) {
+        p_next = k.rows[j_begin];
+        load_pulse<PREC, MONO>(k, tile, j_begin, p_next, d_next);
+        if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j_begin];
+    }
     for (int j = j_begin; j < j_end; ++j) {
-        const int p = k.rows[j];
-        PulseData<MONO> d;
-        load_pulse<PREC, MONO>(k, tile, j, p, d);
+        const int p = p_next;
+        PulseData<MONO> d = d_next;
+        if (j + 1 < j_end) {
+            p_next = k.rows[j + 1];
+            load_pulse<PREC, MONO>(k, tile, j + 1, p_next, d_next);
+            if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j + 1];
+        }
         const float2 *row = k.prof + (size_t)p * k.ld;
         // 16-byte loads of bins (n, n + 1) through a buffer resource over the
         // padded row: offsets beyond it read zeros instead of faulting.
         const __amdgpu_buffer_rsrc_t rsrc =
             __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0, k.ld * 8, kBufferWord3);
-        constexpr bool kShift = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
-        if constexpr (kShift) {
-            // u of the tile centre moved by a multiple of nw (bp_shift_kernel,
-            // once per plan) so that every pixel of the tile lands in
-            // [2, nw + 2 shift_lo - 2): bins are unchanged modulo nw and the
-            // padded row tail makes bins n + 1 <= ld - 1 valid without a wrap.
-            d.s.cu = k.bp_cu[(size_t)tile * k.Pv + j];
-        }
+        if constexpr (INTERP == kInterpLinear && kShift) {
+            // all pixels' bins and phases first, so that their profile loads
+            // are in flight together, then interpolation and accumulation
+            u32x4 h[kPixPerThread];
+            float fr[kPixPerThread], sn[kPixPerThread], cs[kPixPerThread];
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                double u, th;
+                eval_range<PREC, MONO, PLANAR, SORD>(k, d, q[m], u, th, pr);
+                // u > 0: truncation is floor, v_fract_f64 the exact rest
+                const int n0 = (int)u;
+                fr[m] = (float)__builtin_amdgcn_fract(u);
+                h[m] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
+                phasor_fract(th, &sn[m], &cs[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                const float h0x = __uint_as_float(h[m].x), h0y = __uint_as_float(h[m].y);
+                const float hr = fmaf(fr[m], __uint_as_float(h[m].z) - h0x, h0x);
+                const float hi = fmaf(fr[m], __uint_as_float(h[m].w) - h0y, h0y);
+                // h * e^{-i theta}
+                ar[m] = fmaf(hr, cs[m], fmaf(hi, sn[m], ar[m]));
+                ai[m] = fmaf(hi, cs[m], fmaf(-hr, sn[m], ai[m]));
+            }
+        } else {
 #pragma unroll
         for (int m = 0; m < kPixPerThread; ++m) {
             double u, th;
             eval_range<PREC, MONO, PLANAR, SORD>(k, d, q[m], u, th, pr);
             float hr, hi;
             if constexpr (INTERP == kInterpLinear) {
-                int n0;
-                float fr;
-                if constexpr (kShift) {
-                    // u > 0: truncation is floor, v_fract_f64 the exact rest
-                    n0 = (int)u;
-                    fr = (float)__builtin_amdgcn_fract(u);
-                } else {
-                    const double fl = floor(u);
-                    fr = (float)(u - fl);
-                    n0 = bin_index(k, fl, 1);
-                }
+                const double fl = floor(u);
+                const float fr = (float)(u - fl);
+                const int n0 = bin_index(k, fl, 1);
                 const u32x4 h = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
                 const float h0x = __uint_as_float(h.x), h0y = __uint_as_float(h.y);
                 hr = fmaf(fr, __uint_as_float(h.z) - h0x, h0x);
@@ -380,11 +408,11 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
                 hi = h.y;
             }
             float s, c;
-            if constexpr (kShift) phasor_fract(th, &s, &c);
-            else phasor_rev(th, &s, &c);
+            phasor_rev(th, &s, &c);
             // h * e^{-i theta}
             ar[m] = fmaf(hr, c, fmaf(hi, s, ar[m]));
             ai[m] = fmaf(hi, c, fmaf(-hr, s, ai[m]));
+        }
         }
         if (((j - j_begin) & 63) == 63) {
 #pragma unroll
