@@ -834,7 +834,9 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
 constexpr int kRedThreads = 512;
 constexpr int kRedBatch = 8;
 __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
-    extern __shared__ int2 acc[];  // [chunk], then [NR] bases (double)
+    // [chunk] re words, [chunk] im words (separate planes: consecutive bins
+    // fall in consecutive banks), then [NR] bases (double)
+    extern __shared__ int2 acc[];
     __shared__ double red_lo[kRedThreads], red_hi[kRedThreads];
     __shared__ float red_l1[kRedThreads];
     const int j = blockIdx.x;
@@ -873,13 +875,14 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     const float scale = (fixed && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
     if (fixed && scale == 0.f) return;  // every window is zero: the row stays zero
-    float2 *facc = (float2 *)acc;
+    int *acc_re = (int *)acc, *acc_im = acc_re + chunk;
+    float *facc_re = (float *)acc_re, *facc_im = (float *)acc_im;
     float2 *row = k.prof + (size_t)p * k.nw;
     // thread walks elements e = tid + kRedThreads i of [NR][win] without dividing
     const int t_step = kRedThreads / win, l_step = kRedThreads % win;
     const size_t total = (size_t)NR * win;
     for (double c0 = lo; c0 < hi; c0 += (double)chunk) {
-        for (int e = tid; e < chunk; e += kRedThreads) acc[e] = make_int2(0, 0);
+        for (int e = tid; e < chunk; e += kRedThreads) acc_re[e] = acc_im[e] = 0;
         __syncthreads();
         int t = tid / win, l = tid % win;
         for (size_t e0 = tid; e0 < total; e0 += (size_t)kRedThreads * kRedBatch) {
@@ -897,11 +900,11 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
                     if (rel >= 0.0 && rel < (double)chunk) {
                         const int r = (int)rel;
                         if (fixed) {
-                            atomicAdd(&acc[r].x, (int)rintf(v[i].x * scale));
-                            atomicAdd(&acc[r].y, (int)rintf(v[i].y * scale));
+                            atomicAdd(&acc_re[r], (int)rintf(v[i].x * scale));
+                            atomicAdd(&acc_im[r], (int)rintf(v[i].y * scale));
                         } else {
-                            atomicAdd(&facc[r].x, v[i].x);
-                            atomicAdd(&facc[r].y, v[i].y);
+                            atomicAdd(&facc_re[r], v[i].x);
+                            atomicAdd(&facc_im[r], v[i].y);
                         }
                     }
                 }
@@ -919,12 +922,11 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
             const int n = bin_index(k, c0 + (double)e, 1);
             float2 r = row[n];
             if (fixed) {
-                const int2 v = acc[e];
-                r.x += (float)v.x * inv_scale;
-                r.y += (float)v.y * inv_scale;
+                r.x += (float)acc_re[e] * inv_scale;
+                r.y += (float)acc_im[e] * inv_scale;
             } else {
-                r.x += facc[e].x;
-                r.y += facc[e].y;
+                r.x += facc_re[e];
+                r.y += facc_im[e];
             }
             row[n] = r;
         }
