@@ -319,9 +319,12 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const float2 *__re
 }
 
 // Forward: Y[p][dst(k)] = W[k] wfac[p] sum_n prof[p][n] e^{-2 pi i s_k n / N_w}
-// (row stride nw = U N), k over the N band bins.
+// (row stride nw = U N), k over the N band bins.  With `spans`, row p holds
+// data only in the spans[p].y bins from spans[p].x on (wrapping at N_w); the
+// other bins are taken as zero without being read.
 template <int N, int Q>
 __global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__restrict__ prof,
+                                                                const int2 *__restrict__ spans,
                                                                 const float2 *__restrict__ W,
                                                                 const float2 *__restrict__ wfac,
                                                                 int U, int centred,
@@ -336,14 +339,32 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__
     const int nw = U * N;
     const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
     const float2 *row = prof + (size_t)p * nw;
+    const int2 sp = spans ? spans[p] : make_int2(0, nw);
     float2 acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc[q] = make_float2(0.f, 0.f);
     for (int r0 = 0; r0 < U; r0 += Q) {
 #pragma unroll 2
         for (int m = tid; m < N; m += Bt::NT) {
+            const int n = U * m + r0;
+            bool in[Q], any = false;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                int d = n + q - sp.x;
+                d += d < 0 ? nw : 0;
+                in[q] = d < sp.y;
+                any |= in[q];
+            }
             float2 v[Q];
-            load_phases<Q>(row + (size_t)U * m + r0, v);
+            if (any) {
+                load_phases<Q>(row + n, v);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (!in[q]) v[q] = make_float2(0.f, 0.f);
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) v[q] = make_float2(0.f, 0.f);
+            }
 #pragma unroll
             for (int q = 0; q < Q; ++q) lds[q * SZ + phys(m)] = v[q];
         }
@@ -392,13 +413,13 @@ hipError_t up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int U, i
 }
 
 template <int N, int Q>
-hipError_t down(const float2 *prof, const float2 *W, const float2 *wfac, int U, int P, int centred,
-                const float2 *tables, float2 *Y, hipStream_t s) {
+hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac, int U,
+                int P, int centred, const float2 *tables, float2 *Y, hipStream_t s) {
     const size_t lds = lds_bytes<N, Q>(U);
     hipError_t e = hipFuncSetAttribute((const void *)fft_down_kernel<N, Q>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    fft_down_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(prof, W, wfac, U, centred, tables, Y);
+    fft_down_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(prof, spans, W, wfac, U, centred, tables, Y);
     return hipGetLastError();
 }
 
@@ -445,15 +466,16 @@ hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfa
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_fft_down(const float2 *prof, const float2 *W, const float2 *wfac, int N, int U,
-                           int P, int centred, const float2 *tables, float2 *Y, hipStream_t s) {
+hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
+                           int N, int U, int P, int centred, const float2 *tables, float2 *Y,
+                           hipStream_t s) {
     const int Q = fused_fft_phases(N, U, false);
 #define DOWN_CASE(n)                                                                     \
     if (N == n) {                                                                        \
-        if (Q == 8) return down<n, 8>(prof, W, wfac, U, P, centred, tables, Y, s);       \
-        if (Q == 4) return down<n, 4>(prof, W, wfac, U, P, centred, tables, Y, s);       \
-        if (Q == 2) return down<n, 2>(prof, W, wfac, U, P, centred, tables, Y, s);       \
-        return down<n, 1>(prof, W, wfac, U, P, centred, tables, Y, s);                   \
+        if (Q == 8) return down<n, 8>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
+        if (Q == 4) return down<n, 4>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
+        if (Q == 2) return down<n, 2>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
+        return down<n, 1>(prof, spans, W, wfac, U, P, centred, tables, Y, s);                   \
     }
     SARHIP_FFT_SIZES(DOWN_CASE)
 #undef DOWN_CASE

@@ -825,14 +825,16 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
 // One workgroup per valid pulse: merge the pulse's window partials into its
 // profile row.  Window bases and maxima are staged in LDS; the covered span of
 // bins is accumulated in LDS chunks while the partials stream through
-// coalesced loads, kRedBatch independent loads per thread at a time so that
-// enough bytes are in flight to approach HBM bandwidth.  The chunk accumulates
+// coalesced loads: each wave takes whole windows (a window's base is then
+// uniform over the wave), kRedWin windows x kRedSlices 64-bin slices per
+// batch, so that enough bytes are in flight to approach HBM bandwidth.  The chunk accumulates
 // in 32-bit fixed point with LDS integer atomics, scaled by 2^30 / (sum over
 // windows of the window's max |component|), a bound on any bin sum of the
 // row; if that bound is not finite the chunk accumulates in float instead.
-// The chunk is then added into the row.
+// The chunk is then written into the row.
 constexpr int kRedThreads = 512;
-constexpr int kRedBatch = 8;
+constexpr int kRedWin = 4;     // windows per wave per batch
+constexpr int kRedSlices = 2;  // 64-bin slices per window per batch
 __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
     // [chunk] re words, [chunk] im words (separate planes: consecutive bins
     // fall in consecutive banks), then [NR] bases (double)
@@ -874,45 +876,66 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     const bool fixed = isfinite(l1);
     const float scale = (fixed && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
-    if (fixed && scale == 0.f) return;  // every window is zero: the row stays zero
+    // The covered bins [lo, hi) are written (stored) once each, and their span
+    // is recorded for the spectral stage; the rest of the row is not touched.
+    // A span longer than the row wraps onto itself: the row is zeroed first
+    // and the chunks add into it.
+    const bool empty = !(hi > lo) || (fixed && scale == 0.f);  // every window zero
+    const bool full = !empty && hi - lo > (double)k.nw;
+    if (tid == 0)
+        k.fspan[p] = empty ? make_int2(0, 0)
+                           : full ? make_int2(0, k.nw) : make_int2(bin_index(k, lo, 1), (int)(hi - lo));
+    if (empty) return;
     int *acc_re = (int *)acc, *acc_im = acc_re + chunk;
     float *facc_re = (float *)acc_re, *facc_im = (float *)acc_im;
     float2 *row = k.prof + (size_t)p * k.nw;
-    // thread walks elements e = tid + kRedThreads i of [NR][win] without dividing
-    const int t_step = kRedThreads / win, l_step = kRedThreads % win;
-    const size_t total = (size_t)NR * win;
+    if (full) {
+        for (int e = tid; e < k.nw; e += kRedThreads) row[e] = make_float2(0.f, 0.f);
+        __threadfence_block();
+        __syncthreads();
+    }
+    // wave w merges windows w, w + 8, ...: a window's base is uniform over
+    // the wave and its bins are contiguous loads; kRedWin windows per batch
+    // keep several loads in flight per lane
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int kWaves = kRedThreads / 64;
+    const int nl = (win + 63) / 64;  // 64-bin slices per window
     for (double c0 = lo; c0 < hi; c0 += (double)chunk) {
         for (int e = tid; e < chunk; e += kRedThreads) acc_re[e] = acc_im[e] = 0;
         __syncthreads();
-        int t = tid / win, l = tid % win;
-        for (size_t e0 = tid; e0 < total; e0 += (size_t)kRedThreads * kRedBatch) {
-            float2 v[kRedBatch];
+        for (int t0 = wave; t0 < NR; t0 += kWaves * kRedWin) {
+            for (int s0 = 0; s0 < nl; s0 += kRedSlices) {
+                float2 v[kRedWin][kRedSlices];
 #pragma unroll
-            for (int i = 0; i < kRedBatch; ++i) {
-                const size_t e = e0 + (size_t)i * kRedThreads;
-                v[i] = e < total ? part[e] : make_float2(0.f, 0.f);
-            }
+                for (int w = 0; w < kRedWin; ++w) {
+                    const int t = t0 + kWaves * w;
 #pragma unroll
-            for (int i = 0; i < kRedBatch; ++i) {
-                const size_t e = e0 + (size_t)i * kRedThreads;
-                if (e < total) {
-                    const double rel = sb[t] + (double)l - c0;
-                    if (rel >= 0.0 && rel < (double)chunk) {
-                        const int r = (int)rel;
-                        if (fixed) {
-                            atomicAdd(&acc_re[r], (int)rintf(v[i].x * scale));
-                            atomicAdd(&acc_im[r], (int)rintf(v[i].y * scale));
-                        } else {
-                            atomicAdd(&facc_re[r], v[i].x);
-                            atomicAdd(&facc_im[r], v[i].y);
-                        }
+                    for (int sl = 0; sl < kRedSlices; ++sl) {
+                        const int l = (s0 + sl) * 64 + lane;
+                        v[w][sl] = (t < NR && l < win) ? part[(size_t)t * win + l]
+                                                       : make_float2(0.f, 0.f);
                     }
                 }
-                t += t_step;
-                l += l_step;
-                if (l >= win) {
-                    l -= win;
-                    t += 1;
+#pragma unroll
+                for (int w = 0; w < kRedWin; ++w) {
+                    const int t = t0 + kWaves * w;
+                    if (t >= NR) break;
+                    const double base = sb[t] - c0;
+#pragma unroll
+                    for (int sl = 0; sl < kRedSlices; ++sl) {
+                        const int l = (s0 + sl) * 64 + lane;
+                        const double rel = base + (double)l;
+                        if (l < win && rel >= 0.0 && rel < (double)chunk) {
+                            const int r = (int)rel;
+                            if (fixed) {
+                                atomicAdd(&acc_re[r], (int)rintf(v[w][sl].x * scale));
+                                atomicAdd(&acc_im[r], (int)rintf(v[w][sl].y * scale));
+                            } else {
+                                atomicAdd(&facc_re[r], v[w][sl].x);
+                                atomicAdd(&facc_im[r], v[w][sl].y);
+                            }
+                        }
+                    }
                 }
             }
         }
@@ -920,7 +943,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
         const int len = (int)fmin((double)chunk, hi - c0);
         for (int e = tid; e < len; e += kRedThreads) {
             const int n = bin_index(k, c0 + (double)e, 1);
-            float2 r = row[n];
+            float2 r = full ? row[n] : make_float2(0.f, 0.f);
             if (fixed) {
                 r.x += (float)acc_re[e] * inv_scale;
                 r.y += (float)acc_im[e] * inv_scale;

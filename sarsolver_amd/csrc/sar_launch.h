@@ -45,6 +45,7 @@ struct KernelParams {
     float2 *partial;          // [Pv][num_regions][win]
     double *wbases;           // [Pv][num_regions]
     float *wmax;              // [Pv][num_regions] max |component| of each window
+    int2 *fspan;              // [P] covered bins of each forward row: {first bin, count}
     float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
@@ -114,8 +115,10 @@ int fused_fft_phases(int N, int U, bool up);
 hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N, int U,
                          int ld, int P, int centred, const float2 *tables, float2 *prof,
                          hipStream_t s);
-hipError_t launch_fft_down(const float2 *prof, const float2 *W, const float2 *wfac, int N, int U,
-                           int P, int centred, const float2 *tables, float2 *Y, hipStream_t s);
+// spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows
+hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
+                           int N, int U, int P, int centred, const float2 *tables, float2 *Y,
+                           hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

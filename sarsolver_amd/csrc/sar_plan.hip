@@ -136,6 +136,7 @@ struct sarhip_plan_s {
     float2 *d_partial = nullptr;
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
+    int2 *d_fspan = nullptr;  // [P] forward row spans written by the merge
     float *d_region_l1 = nullptr;
     double2 *d_bp_partial = nullptr;
     int bp_splits = 1;
@@ -176,6 +177,7 @@ struct sarhip_plan_s {
         dev_free(d_partial);
         dev_free(d_wbases);
         dev_free(d_wmax);
+        dev_free(d_fspan);
         dev_free(d_region_l1);
         dev_free(d_bp_partial);
         if (fft_work) (void)hipFree(fft_work);
@@ -225,6 +227,7 @@ struct sarhip_plan_s {
         k.partial = d_partial;
         k.wbases = d_wbases;
         k.wmax = d_wmax;
+        k.fspan = d_fspan;
         k.region_l1 = d_region_l1;
         k.bp_splits = bp_splits;
         k.bp_partial = d_bp_partial;
@@ -791,6 +794,9 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     const size_t nrows = std::max<size_t>(1, (size_t)pl->Pv);
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
     HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
+    // rows the merge never writes (invalid pulses) keep the empty span
+    HIP_TRY(hipMalloc((void **)&pl->d_fspan, sizeof(int2) * std::max<size_t>(P, 1)));
+    HIP_TRY(hipMemset(pl->d_fspan, 0, sizeof(int2) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
     // The planar 64-pulse linear scatter streams 32-byte records that the
     // region-norm kernel writes on every forward call.
@@ -1090,7 +1096,11 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     int rc;
     if ((rc = begin_timing(pl, 0))) return rc;
     if ((rc = record(pl, 0, s))) return rc;
-    HIP_TRY(launch_fill_zero(pl->d_prof, sizeof(float2) * pl->P * pl->nw, s));
+    // The merge writes each row's covered span and records it; the fused
+    // spectral stage reads only those spans, so the rows need no clearing.
+    // Global-atomic scatter (no windows) and rocFFT need zeroed rows.
+    const bool spans = pl->fused_U > 0 && pl->Pv > 0 && pl->win > 0;
+    if (!spans) HIP_TRY(launch_fill_zero(pl->d_prof, sizeof(float2) * pl->P * pl->nw, s));
     if (pl->Pv > 0) {
         HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm,
                                    s));
@@ -1107,8 +1117,9 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     if ((rc = record(pl, 1, s))) return rc;
     if (pl->fused_U > 0) {
         // FFT and truncation in one pass over the profile rows
-        HIP_TRY(launch_fft_down(pl->d_prof, pl->d_W, pl->d_wfac_fwd, (int)pl->N, pl->fused_U,
-                                (int)pl->P, order == SARHIP_ORDER_CENTRED, pl->d_fft_tables,
+        HIP_TRY(launch_fft_down(pl->d_prof, spans ? pl->d_fspan : nullptr, pl->d_W, pl->d_wfac_fwd,
+                                (int)pl->N, pl->fused_U, (int)pl->P, order == SARHIP_ORDER_CENTRED,
+                                pl->d_fft_tables,
                                 (float2 *)phase_history, s));
         if ((rc = record(pl, 2, s))) return rc;
         if ((rc = record(pl, 3, s))) return rc;

@@ -83,3 +83,52 @@ def test_fused_phase_batches_against_oracle(dev, oracle_lib, phases):
     ef, ea = rel_l2(Yd, Yo), rel_l2(Xd, Xo)
     print(f"  forward vs oracle {ef:.2e}, adjoint vs oracle {ea:.2e}")
     assert ef <= 1e-5 and ea <= 1e-5
+
+
+@pytest.mark.parametrize("extent", [6.0, 40.0])
+def test_forward_row_spans(dev, oracle_lib, extent):
+    """The merge writes only the bins its windows cover and the fused forward
+    reads only those: a scene whose range extent (~16 bins per metre, N_w =
+    256) covers part of the row, and one that covers it more than twice (the
+    row wraps onto itself), against the oracle and against the rocFFT stage
+    (which zero-fills the rows), on a second call after an adjoint has left
+    other data in the rows."""
+    from sarsolver_amd import synthetic
+    from sarsolver_amd._native import SarPlan
+    from sarsolver_amd.numpy import GridSarScene
+    N, U, P, fs = 64, 4, 40, 1.2e9
+    ap = synthetic.circular_aperture(P, N, 3.0, fs=fs)
+    n = 128  # 16 regions of 32 x 32 pixels, so that each region's window fits the LDS
+    sc = GridSarScene((1, n, n), (0.0, extent, extent))
+    # range extent of the scene in profile bins at the first pulse
+    dr = 2 * np.linalg.norm(sc.positions - ap.transmitter_positions[0], axis=1)
+    bins = np.ptp(dr) * fs * U / ap.c_eff
+    assert (bins > 2 * N * U) if extent > 20 else (bins < N * U)
+    g = np.random.default_rng(int(extent))
+    x = g.standard_normal(n * n) + 1j * g.standard_normal(n * n)
+    W = np.fft.ifftshift(ap.waveform_fft)
+    want = oracle_lib.forward(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
+                              sc.positions, x, W, ap.slow_time_weighting, ap.centre_frequency,
+                              fs, N, upsample_ratio=float(U), interp="linear")
+    xd = torch.from_numpy(x.astype(np.complex64)).to(dev)
+    out = {}
+    for fused in (True, False):
+        os.environ["SARHIP_FUSED_FFT"] = "1" if fused else "0"
+        try:
+            pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
+                         sc.positions, W, ap.slow_time_weighting, ap.centre_frequency, fs, ap.c_eff,
+                         float(U), 1.0, interp="linear", precision="series", device=dev)
+        finally:
+            os.environ.pop("SARHIP_FUSED_FFT", None)
+        assert pl.layout["fused_fft_ratio"] == (U if fused else 0)
+        assert pl.layout["fwd_window_bins"] > 0
+        print(f"case: row spans extent {extent} fused={fused}", flush=True)
+        first = pl.forward(xd).cpu().numpy()
+        pl.adjoint(torch.ones(P, N, dtype=torch.complex64, device=dev))
+        second = pl.forward(xd).cpu().numpy()
+        assert np.array_equal(first, second)
+        out[fused] = second
+        e = rel_l2(second, want)
+        print(f"  vs oracle {e:.2e}")
+        assert e <= 1e-5
+    assert rel_l2(out[True], out[False]) <= 2e-6
