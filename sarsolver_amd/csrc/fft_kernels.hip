@@ -233,10 +233,13 @@ __device__ __forceinline__ void load_phases(const float2 *src, float2 (&v)[Q]) {
 }
 
 // Adjoint: prof[p][n] (row stride ld) = sum_k B[k] e^{2 pi i s_k n / N_w},
-// B[k] = conj(W[k]) wfac[p] Y[p][src(k)]; bins [N_w, ld) repeat bins [0, ld - N_w).
+// B[k] = conj(W[k]) wfac[p] Y[p][src(k)]; bin N_w + j of the tail repeats bin j mod N_w.
 // B is formed once per row, in registers (k = gt + 256 i of each group).
+// With `spans`, only the spans[p].y bins from spans[p].x on (wrapping at N_w)
+// and their tail copies are stored.
 template <int N, int Q>
-__global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const float2 *__restrict__ Y,
+__global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const int2 *__restrict__ spans,
+                                                              const float2 *__restrict__ Y,
                                                               const float2 *__restrict__ Wconj,
                                                               const float2 *__restrict__ wfac,
                                                               int U, int ld, int centred,
@@ -253,6 +256,8 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const float2 *__re
     const float2 wp = wfac[p];
     const float2 *y = Y + (size_t)p * N;
     float2 *row = prof + (size_t)p * ld;
+    const int2 sp = spans ? spans[p] : make_int2(0, nw);
+    if (sp.y == 0) return;  // row never read
     float2 B[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
@@ -306,14 +311,30 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const float2 *__re
         // Q consecutive phases of each m are contiguous in the row
 #pragma unroll 2
         for (int m = tid; m < N; m += Bt::NT) {
-            const size_t n = (size_t)U * m + r0;
+            const int n = U * m + r0;
+            bool in[Q], all = true;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                int d = n + q - sp.x;
+                d += d < 0 ? nw : 0;
+                in[q] = d < sp.y;
+                all &= in[q];
+            }
             float2 v[Q];
 #pragma unroll
             for (int q = 0; q < Q; ++q) v[q] = lds[q * SZ + phys(m)];
-            store_phases<Q>(row + n, v);
+            if (all) {
+                store_phases<Q>(row + n, v);
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (in[q]) row[n + q] = v[q];
+            }
+            // the tail [N_w, ld) can be longer than N_w when tiles are wide
 #pragma unroll
             for (int q = 0; q < Q; ++q)
-                if (n + q < (size_t)(ld - nw)) row[nw + n + q] = v[q];
+                if (in[q])
+                    for (int t = nw + n + q; t < ld; t += nw) row[t] = v[q];
         }
     }
 }
@@ -402,13 +423,14 @@ size_t lds_bytes(int U) {
 }
 
 template <int N, int Q>
-hipError_t up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int U, int ld, int P,
-              int centred, const float2 *tables, float2 *prof, hipStream_t s) {
+hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac, int U,
+              int ld, int P, int centred, const float2 *tables, float2 *prof, hipStream_t s) {
     const size_t lds = lds_bytes<N, Q>(U);
     hipError_t e = hipFuncSetAttribute((const void *)fft_up_kernel<N, Q>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    fft_up_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(Y, Wconj, wfac, U, ld, centred, tables, prof);
+    fft_up_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(spans, Y, Wconj, wfac, U, ld, centred, tables,
+                                                      prof);
     return hipGetLastError();
 }
 
@@ -450,16 +472,16 @@ bool fused_fft_supported(int N, int U) {
 
 #define SARHIP_FFT_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
-hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N, int U,
-                         int ld, int P, int centred, const float2 *tables, float2 *prof,
-                         hipStream_t s) {
+hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
+                         int N, int U, int ld, int P, int centred, const float2 *tables,
+                         float2 *prof, hipStream_t s) {
     const int Q = fused_fft_phases(N, U, true);
-#define UP_CASE(n)                                                                       \
-    if (N == n) {                                                                        \
-        if (Q == 8) return up<n, 8>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        if (Q == 4) return up<n, 4>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        if (Q == 2) return up<n, 2>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        return up<n, 1>(Y, Wconj, wfac, U, ld, P, centred, tables, prof, s);             \
+#define UP_CASE(n)                                                                         \
+    if (N == n) {                                                                          \
+        if (Q == 8) return up<n, 8>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        if (Q == 4) return up<n, 4>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        if (Q == 2) return up<n, 2>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
+        return up<n, 1>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s);             \
     }
     SARHIP_FFT_SIZES(UP_CASE)
 #undef UP_CASE
@@ -470,12 +492,12 @@ hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *
                            int N, int U, int P, int centred, const float2 *tables, float2 *Y,
                            hipStream_t s) {
     const int Q = fused_fft_phases(N, U, false);
-#define DOWN_CASE(n)                                                                     \
-    if (N == n) {                                                                        \
-        if (Q == 8) return down<n, 8>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
-        if (Q == 4) return down<n, 4>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
-        if (Q == 2) return down<n, 2>(prof, spans, W, wfac, U, P, centred, tables, Y, s);       \
-        return down<n, 1>(prof, spans, W, wfac, U, P, centred, tables, Y, s);                   \
+#define DOWN_CASE(n)                                                                   \
+    if (N == n) {                                                                      \
+        if (Q == 8) return down<n, 8>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        if (Q == 4) return down<n, 4>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        if (Q == 2) return down<n, 2>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        return down<n, 1>(prof, spans, W, wfac, U, P, centred, tables, Y, s);             \
     }
     SARHIP_FFT_SIZES(DOWN_CASE)
 #undef DOWN_CASE

@@ -1011,6 +1011,66 @@ __global__ __launch_bounds__(256) void bp_shift_kernel(const double *__restrict_
     out[i] = fma(-floor((cu - shift_lo) * inv_nw), nwd, cu);
 }
 
+// Bins of each adjoint profile row that the backprojection can read, one
+// workgroup per valid pulse: {first bin in [0, nw), count}, the whole row when
+// the count reaches nw.  Per leg |T - X| moves by at most |X - X0| <= r from
+// the tile centre X0 (triangle inequality), so every pixel of the tile has u
+// within 2 r by_dx of u(X0); the row is read at floor(u) to floor(u) + 2
+// (linear neighbours, nearest rules), and 8 bins on each side cover rounding
+// in every precision mode.
+__global__ __launch_bounds__(256) void adjoint_span_kernel(const PulseConst *__restrict__ pc,
+                                                           const int *__restrict__ rows,
+                                                           const double4 *__restrict__ tiles,
+                                                           int nt, int mono, double by_dx, int nw,
+                                                           int2 *__restrict__ span) {
+    __shared__ double s_lo[256], s_hi[256];
+    const int p = rows[blockIdx.x];
+    const int tid = (int)threadIdx.x;
+    const PulseConst c = pc[p];
+    double lo = INFINITY, hi = -INFINITY;
+    for (int t = tid; t < nt; t += 256) {
+        const double4 x = tiles[t];
+        const double ax = c.tx[0] - x.x, ay = c.tx[1] - x.y, az = c.tx[2] - x.z;
+        const double d1 = sqrt(fma(ax, ax, fma(ay, ay, az * az)));
+        double d2 = d1;
+        if (!mono) {
+            const double bx = c.rx[0] - x.x, by = c.rx[1] - x.y, bz = c.rx[2] - x.z;
+            d2 = sqrt(fma(bx, bx, fma(by, by, bz * bz)));
+        }
+        const double u = (d1 + d2 - c.r_srp) * by_dx, du = 2.0 * x.w * by_dx;
+        lo = fmin(lo, u - du);
+        hi = fmax(hi, u + du);
+    }
+    s_lo[tid] = lo;
+    s_hi[tid] = hi;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) {
+            s_lo[tid] = fmin(s_lo[tid], s_lo[tid + h]);
+            s_hi[tid] = fmax(s_hi[tid], s_hi[tid + h]);
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const double first = floor(s_lo[0]) - 8.0;
+        const double count = floor(s_hi[0]) - floor(s_lo[0]) + 19.0;
+        if (!(count < (double)nw) || !isfinite(first)) {
+            span[p] = make_int2(0, nw);
+        } else {
+            const double w = first - floor(first / (double)nw) * (double)nw;
+            span[p] = make_int2(min((int)w, nw - 1), (int)count);
+        }
+    }
+}
+
+hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, const double4 *tiles,
+                                int nt, bool mono, double by_dx, int nw, int2 *span,
+                                hipStream_t s) {
+    if (Pv <= 0) return hipSuccess;
+    adjoint_span_kernel<<<Pv, 256, 0, s>>>(pc, rows, tiles, nt, mono ? 1 : 0, by_dx, nw, span);
+    return hipGetLastError();
+}
+
 hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
                            double nwd, double *out, hipStream_t s) {
     const int stride = mono ? (int)(sizeof(SeriesMono) / 8) : (int)(sizeof(SeriesBi) / 8);

@@ -98,6 +98,10 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
 hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
 hipError_t launch_bp_reduce(const KernelParams &k, hipStream_t s);  // pulse-split images
 hipError_t launch_region_l1(const KernelParams &k, hipStream_t s);  // before launch_fp_bin
+// Per valid pulse, the bins {first, count} of its adjoint row that the
+// backprojection can read (span[p] for p = rows[j]; other entries untouched).
+hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, const double4 *tiles,
+                                int nt, bool mono, double by_dx, int nw, int2 *span, hipStream_t s);
 hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
                            double nwd, double *out, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)
@@ -112,9 +116,10 @@ bool fused_fft_supported(int N, int U);
 int fused_fft_phases(int N, int U, bool up);
 // tables: [64 | N/64 | 256 | U N/256] two-level twiddle tables,
 // e^{-2 pi i q / N}, e^{-2 pi i 64 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
-hipError_t launch_fft_up(const float2 *Y, const float2 *Wconj, const float2 *wfac, int N, int U,
-                         int ld, int P, int centred, const float2 *tables, float2 *prof,
-                         hipStream_t s);
+// spans: per-row bins to store {first, count} (and their tail copies), or NULL for whole rows
+hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
+                         int N, int U, int ld, int P, int centred, const float2 *tables,
+                         float2 *prof, hipStream_t s);
 // spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables, float2 *Y,

@@ -137,6 +137,7 @@ struct sarhip_plan_s {
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
     int2 *d_fspan = nullptr;  // [P] forward row spans written by the merge
+    int2 *d_aspan = nullptr;  // [P] adjoint row spans the backprojection reads (fused path)
     float *d_region_l1 = nullptr;
     double2 *d_bp_partial = nullptr;
     int bp_splits = 1;
@@ -178,6 +179,7 @@ struct sarhip_plan_s {
         dev_free(d_wbases);
         dev_free(d_wmax);
         dev_free(d_fspan);
+        dev_free(d_aspan);
         dev_free(d_region_l1);
         dev_free(d_bp_partial);
         if (fft_work) (void)hipFree(fft_work);
@@ -851,6 +853,12 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
             if ((rc = upload((void **)&pl->d_fft_tables, tab.data(), sizeof(float2) * tab.size())))
                 return rc;
             pl->fused_U = (int)U;
+            // rows of invalid pulses are never read: empty spans
+            HIP_TRY(hipMalloc((void **)&pl->d_aspan, sizeof(int2) * std::max<size_t>(P, 1)));
+            HIP_TRY(hipMemset(pl->d_aspan, 0, sizeof(int2) * std::max<size_t>(P, 1)));
+            HIP_TRY(launch_adjoint_spans(pl->d_pc, pl->d_rows, pl->Pv, pl->d_tile_c,
+                                         pl->num_tiles, mono, pl->by_dx, (int)nw, pl->d_aspan,
+                                         nullptr));
         }
     }
     if (pl->fused_U > 0) {
@@ -1146,8 +1154,8 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     if (pl->fused_U > 0) {
         // band weights, zero padding, inverse FFT and row tails in one pass
         if ((rc = record(pl, 1, s))) return rc;
-        HIP_TRY(launch_fft_up((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
-                              (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
+        HIP_TRY(launch_fft_up(pl->d_aspan, (const float2 *)phase_history, pl->d_Wconj,
+                              pl->d_wfac_adj, (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
                               order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof, s));
     } else {
         HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,

@@ -85,50 +85,69 @@ def test_fused_phase_batches_against_oracle(dev, oracle_lib, phases):
     assert ef <= 1e-5 and ea <= 1e-5
 
 
-@pytest.mark.parametrize("extent", [6.0, 40.0])
-def test_forward_row_spans(dev, oracle_lib, extent):
-    """The merge writes only the bins its windows cover and the fused forward
-    reads only those: a scene whose range extent (~16 bins per metre, N_w =
-    256) covers part of the row, and one that covers it more than twice (the
-    row wraps onto itself), against the oracle and against the rocFFT stage
-    (which zero-fills the rows), on a second call after an adjoint has left
-    other data in the rows."""
+@pytest.mark.parametrize("extent,interp,precision,bistatic", [
+    (6.0, "linear", "series", False), (40.0, "linear", "series", False),
+    (6.0, "nearest_ref", "fp64", False), (6.0, "linear", "series", True),
+    (40.0, "nearest", "series", True)])
+def test_row_spans(dev, oracle_lib, extent, interp, precision, bistatic):
+    """The fused stages touch only part of each profile row: the forward merge
+    stores the bins its windows cover and the forward spectral stage reads only
+    those; the adjoint spectral stage stores only the bins the backprojection
+    can read.  A scene whose range extent (~16 bins per metre, N_w = 256)
+    covers part of the row and one that covers it more than twice (the row
+    wraps onto itself), monostatic and bistatic, against the oracle and
+    against the rocFFT stages (which write and read whole rows); each
+    direction is called again after calls with other data have left their
+    values in the rows, and must repeat its first result exactly."""
     from sarsolver_amd import synthetic
     from sarsolver_amd._native import SarPlan
     from sarsolver_amd.numpy import GridSarScene
     N, U, P, fs = 64, 4, 40, 1.2e9
     ap = synthetic.circular_aperture(P, N, 3.0, fs=fs)
+    tx, srp = ap.transmitter_positions, ap.stab_ref_posns
+    rx = tx + np.array([300.0, -800.0, 100.0]) if bistatic else ap.receiver_positions
     n = 128  # 16 regions of 32 x 32 pixels, so that each region's window fits the LDS
     sc = GridSarScene((1, n, n), (0.0, extent, extent))
     # range extent of the scene in profile bins at the first pulse
-    dr = 2 * np.linalg.norm(sc.positions - ap.transmitter_positions[0], axis=1)
+    dr = np.linalg.norm(sc.positions - tx[0], axis=1) + np.linalg.norm(sc.positions - rx[0], axis=1)
     bins = np.ptp(dr) * fs * U / ap.c_eff
     assert (bins > 2 * N * U) if extent > 20 else (bins < N * U)
     g = np.random.default_rng(int(extent))
     x = g.standard_normal(n * n) + 1j * g.standard_normal(n * n)
+    y = g.standard_normal((P, N)) + 1j * g.standard_normal((P, N))
     W = np.fft.ifftshift(ap.waveform_fft)
-    want = oracle_lib.forward(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
-                              sc.positions, x, W, ap.slow_time_weighting, ap.centre_frequency,
-                              fs, N, upsample_ratio=float(U), interp="linear")
+    args = (W, ap.slow_time_weighting, ap.centre_frequency, fs)
+    want_y = oracle_lib.forward(tx, rx, srp, sc.positions, x, *args, N, upsample_ratio=float(U),
+                                interp=interp)
+    want_x = oracle_lib.adjoint(tx, rx, srp, sc.positions, y, *args, upsample_ratio=float(U),
+                                interp=interp)
     xd = torch.from_numpy(x.astype(np.complex64)).to(dev)
+    yd = torch.from_numpy(y.astype(np.complex64)).to(dev)
+    other_x = 1e3 * torch.randn(n * n, dtype=torch.complex64, device=dev)
+    other_y = 1e3 * torch.randn(P, N, dtype=torch.complex64, device=dev)
     out = {}
     for fused in (True, False):
         os.environ["SARHIP_FUSED_FFT"] = "1" if fused else "0"
         try:
-            pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
-                         sc.positions, W, ap.slow_time_weighting, ap.centre_frequency, fs, ap.c_eff,
-                         float(U), 1.0, interp="linear", precision="series", device=dev)
+            pl = SarPlan(tx, rx, srp, sc.positions, W, ap.slow_time_weighting, ap.centre_frequency,
+                         fs, ap.c_eff, float(U), 1.0, interp=interp, precision=precision,
+                         device=dev)
         finally:
             os.environ.pop("SARHIP_FUSED_FFT", None)
         assert pl.layout["fused_fft_ratio"] == (U if fused else 0)
         assert pl.layout["fwd_window_bins"] > 0
-        print(f"case: row spans extent {extent} fused={fused}", flush=True)
-        first = pl.forward(xd).cpu().numpy()
-        pl.adjoint(torch.ones(P, N, dtype=torch.complex64, device=dev))
-        second = pl.forward(xd).cpu().numpy()
-        assert np.array_equal(first, second)
-        out[fused] = second
-        e = rel_l2(second, want)
-        print(f"  vs oracle {e:.2e}")
-        assert e <= 1e-5
-    assert rel_l2(out[True], out[False]) <= 2e-6
+        print(f"case: row spans extent {extent} {interp} {precision} bistatic={bistatic} "
+              f"fused={fused}", flush=True)
+        f1 = pl.forward(xd).cpu().numpy()
+        a1 = pl.adjoint(yd).cpu().numpy()
+        pl.adjoint(other_y)
+        f2 = pl.forward(xd).cpu().numpy()
+        pl.forward(other_x)
+        a2 = pl.adjoint(yd).cpu().numpy()
+        assert np.array_equal(f1, f2) and np.array_equal(a1, a2)
+        out[fused] = (f2, a2)
+        ef, ea = rel_l2(f2, want_y), rel_l2(a2, want_x)
+        print(f"  vs oracle: forward {ef:.2e}, adjoint {ea:.2e}")
+        assert ef <= 1e-5 and ea <= 1e-5
+    assert rel_l2(out[True][0], out[False][0]) <= 2e-6
+    assert rel_l2(out[True][1], out[False][1]) <= 2e-6
