@@ -465,12 +465,13 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 //
 // Window bins accumulate in 32-bit fixed point with LDS integer atomics: on
 // gfx950 ds_add_u32 runs ~40x faster than ds_add_f32.  The scale is
-// 2^30 / sum_region(|Re a| + |Im a|): no bin sum can exceed the region's L1
-// norm (interpolation weights are at most 1), so nothing overflows; FP32
-// products carry ~6e-8 relative error and the rounding to integers adds at
-// most 2^-31 of the norm.  The integer parts of the two linear-interpolation
-// shares sum to the rounded total.  A region whose amplitudes are not finite
-// uses float global atomics.
+// 2^31 / (1.001 sum_region(|Re a| + |Im a|)): no bin sum can exceed the
+// region's L1 norm (interpolation weights are at most 1) and the conversion
+// to integers truncates toward zero, which never increases a magnitude, so
+// every sum stays inside int32; FP32 products carry ~6e-8 relative error and
+// the truncation adds less than 2^-31 of the norm per contribution.  The
+// integer parts of the two linear-interpolation shares sum to the truncated
+// total.  A region whose amplitudes are not finite uses float global atomics.
 //
 // Windows are converted back to float and written whole (zeros included) to
 // `partial` with their base bins and max |component|, then merged per pulse
@@ -478,15 +479,16 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 // does not depend on scheduling.  A contribution outside its window (not
 // expected) falls back to a global atomic into the profile.
 // ---------------------------------------------------------------------------
-// Nearest integer, ties to even.  The scaled values carry only a few
-// fractional bits (values near 2^20 keep 4), so exact halves are common:
-// rounding them always up (floor(v + 0.5), one v_cvt_rpi_i32_f32) biases every
-// bin sum by about n/32 units and was measurably worse on dense scenes.
-__device__ __forceinline__ int fix_rn(const float v) { return (int)rintf(v); }
-// Fixed-point scale of a region with L1 norm l1: 2^30 / (1.001 l1), so that no
-// bin sum of the region reaches 2^30 (interpolation weights are <= 1).
+// Toward zero (one v_cvt_i32_f32).  Its error is odd in v, so bin sums of
+// contributions with mixed signs carry no bias; rounding halves always up
+// (floor(v + 0.5)) biased every sum by about n/32 units, since the scaled
+// values keep only a few fractional bits.  At the 2^31 scale the truncation
+// error matches that of round-to-nearest at 2^30.
+__device__ __forceinline__ int fix_tz(const float v) { return (int)v; }
+// Fixed-point scale of a region with L1 norm l1: 2^31 / (1.001 l1), so that no
+// bin sum of the region reaches 2^31 / 1.001 (interpolation weights are <= 1).
 __device__ __forceinline__ float fixed_scale(const float l1) {
-    return 1073741824.0f / (l1 * 1.001f);
+    return 2147483648.0f / (l1 * 1.001f);
 }
 // Read-only element at a wave-uniform index, through the constant address
 // space so that the compiler may use scalar loads.
@@ -700,7 +702,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     // forward bin rule round(u) for both nearest modes (CPP:289)
                     fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
                 }
-                phasor_rev(th, &sn, &cs);
+                phasor_fract(th, &sn, &cs);
                 // contribution a e^{i theta} (times `scale` when pre-scaled)
                 const float cr = fmaf(cs, a.x, -(sn * a.y));
                 const float ci = fmaf(cs, a.y, sn * a.x);
@@ -709,8 +711,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     const float fr = kRel ? (float)__builtin_amdgcn_fract(u) : (float)(u - fl);
                     if (fixed_ok && (unsigned)l < (unsigned)(win - 1)) {
                         if (qok) {
-                            const int tr = fix_rn(cr), ti = fix_rn(ci);
-                            const int r1 = fix_rn(fr * cr), i1 = fix_rn(fr * ci);
+                            const int tr = fix_tz(cr), ti = fix_tz(ci);
+                            const int r1 = fix_tz(fr * cr), i1 = fix_tz(fr * ci);
                             add_fix(ws + 2 * l, tr - r1, ti - i1);
                             add_fix(ws + 2 * l + 2, r1, i1);
                         }
@@ -725,7 +727,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     }
                 } else {
                     if (fixed_ok && (unsigned)l < (unsigned)win) {
-                        if (qok) add_fix(ws + 2 * l, fix_rn(cr), fix_rn(ci));
+                        if (qok) add_fix(ws + 2 * l, fix_tz(cr), fix_tz(ci));
                     } else if (qok) {
                         const int nb = bin_index(k, fl, 1);
                         atomicAdd(&row[nb].x, cr * unscale);
@@ -827,10 +829,12 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
 // bins is accumulated in LDS chunks while the partials stream through
 // coalesced loads: each wave takes whole windows (a window's base is then
 // uniform over the wave), kRedWin windows x kRedSlices 64-bin slices per
-// batch, so that enough bytes are in flight to approach HBM bandwidth.  The chunk accumulates
-// in 32-bit fixed point with LDS integer atomics, scaled by 2^30 / (sum over
-// windows of the window's max |component|), a bound on any bin sum of the
-// row; if that bound is not finite the chunk accumulates in float instead.
+// batch, so that enough bytes are in flight to approach HBM bandwidth.  The
+// chunk accumulates in 32-bit fixed point with LDS integer atomics, scaled by
+// 2^31 / (1.001 x the sum over windows of the window's max |component|), a
+// bound on any bin sum of the row (rounding adds at most half a unit per
+// window, far inside the 0.1% margin); if that bound is not finite the chunk
+// accumulates in float.
 // The chunk is then written into the row.
 constexpr int kRedThreads = 512;
 constexpr int kRedWin = 4;     // windows per wave per batch
@@ -874,7 +878,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     hi = red_hi[0] + (double)win;
     l1 = red_l1[0];
     const bool fixed = isfinite(l1);
-    const float scale = (fixed && l1 > 0.f) ? 1073741824.0f / (l1 * 1.001f) : 0.f;
+    const float scale = (fixed && l1 > 0.f) ? 2147483648.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
     // The covered bins [lo, hi) are written (stored) once each, and their span
     // is recorded for the spectral stage; the rest of the row is not touched.
