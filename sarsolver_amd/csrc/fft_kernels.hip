@@ -448,13 +448,17 @@ hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const fl
 }  // namespace
 
 // Phases per batch: the largest Q of 8, 4, 2, 1 that divides U, whose Q
-// buffers and the tables fit the LDS, and not above a cap: 8 for the adjoint
-// (contiguous stores matter most), 2 for the forward (measured at N = 4096,
-// U = 8: 0.39 ms against 0.41 ms with 4); SARHIP_FFT_PHASES overrides both.
-int fused_fft_phases(int N, int U, bool up) {
+// buffers and the tables fit the LDS, and not above a cap.  When the rows are
+// read or written whole, batches make the row accesses contiguous: cap 8 for
+// the adjoint, 2 for the forward (N = 4096, U = 8: 0.39 ms against 0.41 ms
+// with 4).  When only sparse spans of the rows are touched the transforms
+// dominate and occupancy wins: cap 1 (0.19 / 0.21 ms against 0.24 / 0.23 ms
+// with the dense caps, spans of ~9% of the row).  SARHIP_FFT_PHASES
+// overrides every cap.
+int fused_fft_phases(int N, int U, bool up, bool sparse) {
     const size_t per = sizeof(float2) * (size_t)(N + N / 16);
     const size_t tab = sizeof(float2) * (64 + N / 64 + 256 + (size_t)U * N / 256);
-    int cap = up ? 8 : 2;
+    int cap = sparse ? 1 : up ? 8 : 2;
     if (const char *e = std::getenv("SARHIP_FFT_PHASES")) cap = std::atoi(e);
     for (int Q : {8, 4, 2}) {
         if (Q <= cap && U % Q == 0 && Q * per + tab <= 160 * 1024) return Q;
@@ -474,8 +478,8 @@ bool fused_fft_supported(int N, int U) {
 
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
-                         float2 *prof, hipStream_t s) {
-    const int Q = fused_fft_phases(N, U, true);
+                         float2 *prof, bool sparse, hipStream_t s) {
+    const int Q = fused_fft_phases(N, U, true, sparse);
 #define UP_CASE(n)                                                                         \
     if (N == n) {                                                                          \
         if (Q == 8) return up<n, 8>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
@@ -490,8 +494,8 @@ hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj
 
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables, float2 *Y,
-                           hipStream_t s) {
-    const int Q = fused_fft_phases(N, U, false);
+                           bool sparse, hipStream_t s) {
+    const int Q = fused_fft_phases(N, U, false, sparse);
 #define DOWN_CASE(n)                                                                   \
     if (N == n) {                                                                      \
         if (Q == 8) return down<n, 8>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \

@@ -113,17 +113,19 @@ hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, con
                                 float2 *out, hipStream_t s);
 // fused spectral stages (fft_kernels.hip): N-sample band <-> U N-sample rows
 bool fused_fft_supported(int N, int U);
-int fused_fft_phases(int N, int U, bool up);
+int fused_fft_phases(int N, int U, bool up, bool sparse);
 // tables: [64 | N/64 | 256 | U N/256] two-level twiddle tables,
 // e^{-2 pi i q / N}, e^{-2 pi i 64 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
-// spans: per-row bins to store {first, count} (and their tail copies), or NULL for whole rows
+// spans: per-row bins to store {first, count} (and their tail copies), or NULL for whole rows;
+// sparse: the spans cover a small part of the rows (picks the phase batching)
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
-                         float2 *prof, hipStream_t s);
-// spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows
+                         float2 *prof, bool sparse, hipStream_t s);
+// spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows;
+// sparse: as for launch_fft_up
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables, float2 *Y,
-                           hipStream_t s);
+                           bool sparse, hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

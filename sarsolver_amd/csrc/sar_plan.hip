@@ -127,6 +127,7 @@ struct sarhip_plan_s {
     PixRec *d_pixrec = nullptr;
     void *d_series = nullptr;
     int fused_U = 0;                                // > 0: fused spectral kernels, N_w = U N
+    bool fft_sparse = false;  // adjoint spans cover under half of the valid rows' bins
     float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
@@ -859,6 +860,11 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
             HIP_TRY(launch_adjoint_spans(pl->d_pc, pl->d_rows, pl->Pv, pl->d_tile_c,
                                          pl->num_tiles, mono, pl->by_dx, (int)nw, pl->d_aspan,
                                          nullptr));
+            std::vector<int2> sp(P);
+            HIP_TRY(hipMemcpy(sp.data(), pl->d_aspan, sizeof(int2) * P, hipMemcpyDeviceToHost));
+            double covered = 0.0;
+            for (const int2 &v : sp) covered += v.y;
+            pl->fft_sparse = covered < 0.5 * (double)pl->Pv * (double)nw;
         }
     }
     if (pl->fused_U > 0) {
@@ -1127,8 +1133,8 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
         // FFT and truncation in one pass over the profile rows
         HIP_TRY(launch_fft_down(pl->d_prof, spans ? pl->d_fspan : nullptr, pl->d_W, pl->d_wfac_fwd,
                                 (int)pl->N, pl->fused_U, (int)pl->P, order == SARHIP_ORDER_CENTRED,
-                                pl->d_fft_tables,
-                                (float2 *)phase_history, s));
+                                pl->d_fft_tables, (float2 *)phase_history,
+                                spans && pl->fft_sparse, s));
         if ((rc = record(pl, 2, s))) return rc;
         if ((rc = record(pl, 3, s))) return rc;
         return SARHIP_OK;
@@ -1156,7 +1162,8 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
         if ((rc = record(pl, 1, s))) return rc;
         HIP_TRY(launch_fft_up(pl->d_aspan, (const float2 *)phase_history, pl->d_Wconj,
                               pl->d_wfac_adj, (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
-                              order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof, s));
+                              order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof,
+                              pl->fft_sparse, s));
     } else {
         HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
                                   (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,
