@@ -237,14 +237,17 @@ __device__ __forceinline__ void load_phases(const float2 *src, float2 (&v)[Q]) {
 // B is formed once per row, in registers (k = gt + 256 i of each group).
 // With `spans`, only the spans[p].y bins from spans[p].x on (wrapping at N_w)
 // and their tail copies are stored.
+// Waves per SIMD the register allocation must allow: 3 (168 VGPRs) where
+// that needs no spills (N <= 4096, Q <= 2; measured 0.187 -> 0.172 ms at
+// N = 4096), else no constraint.
 template <int N, int Q>
-__global__ __launch_bounds__(Batch<Q>::NT) void fft_up_kernel(const int2 *__restrict__ spans,
-                                                              const float2 *__restrict__ Y,
-                                                              const float2 *__restrict__ Wconj,
-                                                              const float2 *__restrict__ wfac,
-                                                              int U, int ld, int centred,
-                                                              const float2 *__restrict__ tables,
-                                                              float2 *__restrict__ prof) {
+constexpr int kUpWaves = (N <= 4096 && Q <= 2) ? 3 : 1;
+
+template <int N, int Q>
+__global__ __launch_bounds__(Batch<Q>::NT) __attribute__((amdgpu_waves_per_eu(kUpWaves<N, Q>)))
+void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
+                   const float2 *__restrict__ Wconj, const float2 *__restrict__ wfac, int U, int ld,
+                   int centred, const float2 *__restrict__ tables, float2 *__restrict__ prof) {
     using Bt = Batch<Q>;
     extern __shared__ float2 lds[];
     constexpr int SZ = N + N / 16;
