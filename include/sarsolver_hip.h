@@ -262,6 +262,20 @@ size_t sarhip_conv3x3_wgrad_workspace(int B, int H, int W, int out_c);
 int sarhip_conv3x3_wgrad(const void *x, const float *dy, int B, int H, int W, int out_c,
                          float *dwdb, void *workspace, size_t workspace_bytes, void *stream);
 
+/* Inference of the preconditioner network in one call (no counterpart in the
+ * reference): for each image b of z [B][H*W] complex64, s_b = max(sqrt(mean
+ * |z_b|^2), 1e-30) (FP64 partial sums in a fixed order), x = bf16(z_b / s_b) in
+ * channels 0 and 1 of a 32-channel input, `layers` convolutions as
+ * sarhip_conv3x3_bf16 (wfrags[i], biases[i]: device pointers in host arrays;
+ * out_c = 32, ..., 32, 2; ReLU after all but the last, the last in FP32), and
+ * out_b = (z_b + s_b * (net_0 + i net_1)) * demod (demod [H*W] complex64, or
+ * NULL for none).  `work`: device workspace of at least
+ * sarhip_precond_net_workspace(B, H, W) bytes.  out may not alias z. */
+size_t sarhip_precond_net_workspace(int B, int H, int W);
+int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, int layers,
+                       const void *const *wfrags, const float *const *biases, const int *out_c,
+                       void *work, size_t work_bytes, void *out, void *stream);
+
 /* Message describing the most recent convolution failure on the calling thread. */
 const char *sarhip_conv_last_error(void);
 

@@ -169,6 +169,101 @@ __global__ __launch_bounds__(256) void conv_wgrad_reduce_kernel(const float *__r
     out[e] = (float)s;
 }
 
+// ---- preconditioner network inference in one call --------------------------
+// Per image b of complex z [B][n]: s_b = max(sqrt(mean |z_b|^2), 1e-30) from
+// FP64 partial sums in a fixed order; the network input is bf16(z / s) in
+// channels 0 and 1 (others zero); the output is (z + s net(z / s)) times an
+// optional per-pixel phasor.
+constexpr int kRmsChunk = 4096;  // elements per partial sum
+
+__global__ __launch_bounds__(256) void rms_partial_kernel(const float2 *__restrict__ z, size_t n,
+                                                          int nchunk, double *__restrict__ part) {
+    __shared__ double red[256];
+    const int b = blockIdx.y, t = threadIdx.x;
+    const size_t base = (size_t)blockIdx.x * kRmsChunk;
+    const float2 *zb = z + (size_t)b * n;
+    double acc = 0.0;
+    for (int i = t; i < kRmsChunk; i += 256) {
+        const size_t e = base + i;
+        if (e < n) {
+            const float2 v = zb[e];
+            acc += (double)v.x * v.x + (double)v.y * v.y;
+        }
+    }
+    red[t] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) red[t] += red[t + h];
+        __syncthreads();
+    }
+    if (t == 0) part[(size_t)b * nchunk + blockIdx.x] = red[0];
+}
+
+__device__ __forceinline__ float image_rms(const double *part, int b, int nchunk, size_t n) {
+    double acc = 0.0;
+    for (int c = 0; c < nchunk; ++c) acc += part[(size_t)b * nchunk + c];
+    return fmaxf((float)sqrt(acc / (double)n), 1e-30f);
+}
+
+// x [B][n][32] bf16: channels 0, 1 = z / s, the rest zero; s_out[b] = s.
+__global__ __launch_bounds__(256) void precond_pack_kernel(const float2 *__restrict__ z, size_t n,
+                                                           int nchunk, const double *__restrict__ part,
+                                                           float *__restrict__ s_out,
+                                                           uint16_t *__restrict__ x) {
+    __shared__ float s_sh;
+    const int b = blockIdx.y;
+    if (threadIdx.x == 0) {
+        s_sh = image_rms(part, b, nchunk, n);
+        if (blockIdx.x == 0) s_out[b] = s_sh;
+    }
+    __syncthreads();
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float s = s_sh;
+    const float2 v = z[(size_t)b * n + i];
+    const uint32_t w0 = (uint32_t)f32_to_bf16(v.x / s) | ((uint32_t)f32_to_bf16(v.y / s) << 16);
+    uint4 *dst = (uint4 *)(x + ((size_t)b * n + i) * kC);
+    dst[0] = make_uint4(w0, 0u, 0u, 0u);
+    dst[1] = dst[2] = dst[3] = make_uint4(0u, 0u, 0u, 0u);
+}
+
+// out[b][i] = (z + s_b (y0 + i y1)) * demod[i] (demod optional), y [B][n][2] f32.
+__global__ __launch_bounds__(256) void precond_unpack_kernel(const float2 *__restrict__ z,
+                                                             const float *__restrict__ y,
+                                                             const float *__restrict__ s_in,
+                                                             const float2 *__restrict__ demod,
+                                                             size_t n, float2 *__restrict__ out) {
+    const int b = blockIdx.y;
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const size_t e = (size_t)b * n + i;
+    const float s = s_in[b];
+    const float2 v = z[e];
+    const float2 yy = *(const float2 *)(y + 2 * e);
+    float2 r = make_float2(fmaf(s, yy.x, v.x), fmaf(s, yy.y, v.y));
+    if (demod) {
+        const float2 d = demod[i];
+        r = make_float2(fmaf(r.x, d.x, -r.y * d.y), fmaf(r.x, d.y, r.y * d.x));
+    }
+    out[e] = r;
+}
+
+struct NetWork {
+    size_t act, y, part, s, total;  // byte offsets
+};
+NetWork net_workspace(int B, int H, int W) {
+    auto up = [](size_t v) { return (v + 255) / 256 * 256; };
+    const size_t n = (size_t)H * W;
+    const size_t nchunk = (n + kRmsChunk - 1) / kRmsChunk;
+    NetWork w;
+    w.act = 0;
+    w.y = up(2 * (size_t)B * n * kC * sizeof(uint16_t));
+    w.part = w.y + up((size_t)B * n * 2 * sizeof(float));
+    w.s = w.part + up((size_t)B * nchunk * sizeof(double));
+    w.total = w.s + up((size_t)B * sizeof(float));
+    return w;
+}
+
 thread_local std::string g_conv_err;
 
 }  // namespace
@@ -239,6 +334,64 @@ int sarhip_conv3x3_wgrad(const void *x, const float *dy, int B, int H, int W, in
     conv_wgrad_reduce_kernel<<<(nelem + 255) / 256, 256, 0, s>>>((const float *)workspace, tiles,
                                                                 nelem, dwdb);
     const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_conv_err = hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+size_t sarhip_precond_net_workspace(int B, int H, int W) {
+    if (B <= 0 || H <= 0 || W <= 0) return 0;
+    return net_workspace(B, H, W).total;
+}
+
+int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, int layers,
+                       const void *const *wfrags, const float *const *biases, const int *out_c,
+                       void *work, size_t work_bytes, void *out, void *stream) {
+    if (!z || !out || !wfrags || !biases || !out_c || !work || B <= 0 || H <= 0 || W <= 0 ||
+        layers < 1 || (size_t)B * H * W > ((size_t)1 << 31)) {
+        g_conv_err = "sarhip_precond_net: bad argument";
+        return -1;
+    }
+    for (int i = 0; i < layers; ++i) {
+        if (!wfrags[i] || !biases[i] || out_c[i] != (i == layers - 1 ? 2 : kC)) {
+            g_conv_err = "sarhip_precond_net: layer widths must be 32, ..., 32, 2";
+            return -1;
+        }
+    }
+    const NetWork nw = net_workspace(B, H, W);
+    if (work_bytes < nw.total) {
+        g_conv_err = "sarhip_precond_net: workspace too small";
+        return -1;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    char *wk = (char *)work;
+    const size_t n = (size_t)H * W;
+    const int nchunk = (int)((n + kRmsChunk - 1) / kRmsChunk);
+    uint16_t *act[2] = {(uint16_t *)(wk + nw.act), (uint16_t *)(wk + nw.act) + (size_t)B * n * kC};
+    float *y = (float *)(wk + nw.y);
+    double *part = (double *)(wk + nw.part);
+    float *sv = (float *)(wk + nw.s);
+    const float2 *zz = (const float2 *)z;
+    rms_partial_kernel<<<dim3(nchunk, B), 256, 0, st>>>(zz, n, nchunk, part);
+    const dim3 pgrid((unsigned)((n + 255) / 256), B);
+    precond_pack_kernel<<<pgrid, 256, 0, st>>>(zz, n, nchunk, part, sv, act[0]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        g_conv_err = hipGetErrorString(e);
+        return -2;
+    }
+    for (int i = 0; i < layers; ++i) {
+        const bool last = i == layers - 1;
+        void *dst = last ? (void *)y : (void *)act[(i + 1) & 1];
+        const int rc = sarhip_conv3x3_bf16(act[i & 1], wfrags[i], biases[i], B, H, W, out_c[i],
+                                           last ? 0 : 1, last ? 1 : 0, dst, stream);
+        if (rc != 0) return rc;
+    }
+    precond_unpack_kernel<<<pgrid, 256, 0, st>>>(zz, y, sv, (const float2 *)demod, n,
+                                                 (float2 *)out);
+    e = hipGetLastError();
     if (e != hipSuccess) {
         g_conv_err = hipGetErrorString(e);
         return -2;

@@ -61,6 +61,11 @@ def _lib():
         L.sarhip_conv3x3_wgrad_workspace.argtypes = [ctypes.c_int] * 4
         L.sarhip_conv3x3_wgrad_workspace.restype = ctypes.c_size_t
         L.sarhip_conv_last_error.restype = ctypes.c_char_p
+        L.sarhip_precond_net_workspace.argtypes = [ctypes.c_int] * 3
+        L.sarhip_precond_net_workspace.restype = ctypes.c_size_t
+        L.sarhip_precond_net.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
+        L.sarhip_precond_net.restype = ctypes.c_int
         L._conv_bound = True
     return L
 
@@ -222,14 +227,47 @@ class CnnPreconditioner:
                 x = conv3x3(x, packs[i], biases[i], self.weights[i].shape[2], not last, last)
         return torch.complex(x[..., 0], x[..., 1])
 
+    def _fused(self) -> bool:
+        widths = [w.shape[2] for w in self.weights]
+        return all(c == C for c in widths[:-1]) and widths[-1] == 2
+
+    def _infer(self, z: torch.Tensor, demod: Optional[torch.Tensor]) -> torch.Tensor:
+        """Inference in one library call (sarhip_precond_net): z [B, H, W]
+        complex64 -> (z + s net(z / s)) * demod."""
+        B, H, W = z.shape
+        L = _lib()
+        packs, biases = self._packed()
+        key = (B, H, W, z.device)
+        if getattr(self, "_work_key", None) != key:
+            nbytes = int(L.sarhip_precond_net_workspace(B, H, W))
+            self._work = torch.empty(nbytes, dtype=torch.uint8, device=z.device)
+            self._work_key = key
+        n = len(packs)
+        vp = ctypes.c_void_p
+        wf = (vp * n)(*[p.data_ptr() for p in packs])
+        bs = (vp * n)(*[b.data_ptr() for b in biases])
+        oc = (ctypes.c_int * n)(*[w.shape[2] for w in self.weights])
+        out = torch.empty_like(z)
+        rc = L.sarhip_precond_net(_ptr(z), None if demod is None else _ptr(demod), B, H, W, n,
+                                  ctypes.cast(wf, vp), ctypes.cast(bs, vp), ctypes.cast(oc, vp),
+                                  _ptr(self._work), self._work.numel(), _ptr(out),
+                                  _stream(z.device))
+        _conv_check(rc, "sarhip_precond_net")
+        return out
+
     def __call__(self, r: torch.Tensor, shape) -> torch.Tensor:
         """Apply to a flat residual [M] (or [B, M]) of images of `shape` (H, W).
         With a base preconditioner, the network refines its output in the
-        base's demodulated coordinates."""
+        base's demodulated coordinates.  Without gradients the network runs as
+        one library call (scale, bf16 packing, the layers, output and phasor)."""
         flat = r.dim() == 1
         b = self.base
         zb = r if b is None else b.apply_base(b.to_base(r))
         z = zb.reshape((-1,) + tuple(shape))
+        if not torch.is_grad_enabled() and self._fused():
+            demod = None if b is None else b.demod
+            out = self._infer(z.to(torch.complex64).contiguous(), demod)
+            return out.reshape(-1) if flat else out.reshape(r.shape)
         s = torch.sqrt(torch.mean(torch.abs(z) ** 2, dim=(1, 2), keepdim=True)).clamp_min(1e-30)
         out = (z + s * self.net(z / s)).reshape(r.shape)
         if b is not None:

@@ -149,3 +149,38 @@ def test_learned_preconditioner_gain(dev):
         assert v["max_true_residual"] <= 1.05e-3
     assert l1 <= l0 + 0.05
     assert s["learned"]["median_iters"] <= s["cgls"]["median_iters"]
+
+
+@pytest.mark.parametrize("with_base,batch", [(False, 1), (True, 1), (True, 2)])
+def test_fused_inference_matches_layer_path(dev, with_base, batch):
+    """sarhip_precond_net (scale, bf16 packing, six layers, output and phasor
+    in one call) against the layer-by-layer path the training uses, with
+    random weights in every layer (the last one included), on 128^2 images,
+    flat and batched, with and without the demodulated circulant base."""
+    from sarsolver_amd import learned
+    from sarsolver_amd.train_preconditioner import workload_operator
+    w, op = workload_operator("c1_small", dev)
+    n = w.n
+    base = learned.CirculantPreconditioner(op, n, 0.08, pad=2 * n) if with_base else None
+    pre = learned.CnnPreconditioner(dev, base=base, seed=3)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    with torch.no_grad():
+        pre.weights[-1].copy_(0.05 * torch.randn(pre.weights[-1].shape, generator=g))
+        for b in pre.biases:
+            b.copy_(0.02 * torch.randn(b.shape, generator=g))
+    shape = (n * n,) if batch == 1 else (batch, n * n)
+    r = torch.complex(torch.randn(shape, generator=g), torch.randn(shape, generator=g)).to(dev)
+    print(f"case: fused preconditioner inference base={with_base} batch={batch}", flush=True)
+    assert pre._fused()
+    with torch.no_grad():
+        fused = pre(r, (n, n))
+    with torch.enable_grad():
+        ref = pre(r, (n, n)).detach()
+    assert fused.shape == ref.shape and fused.dtype == ref.dtype
+    z = r if base is None else base.apply_base(base.to_base(r))
+    e = rel_l2(fused.cpu().numpy(), ref.cpu().numpy())
+    # the network's share of the output, to show the comparison is not of z alone
+    share = rel_l2((ref if base is None else base.to_base(ref)).cpu().numpy(), z.cpu().numpy())
+    print(f"  fused vs layer path {e:.2e} (network share of the output {share:.2e})")
+    assert share > 1e-2
+    assert e <= 2e-3 * max(share, 1.0)
