@@ -113,11 +113,15 @@ def test_series_falls_back_when_unbounded(dev, little):
     print("little series layout:", pl.layout)
 
 
-def test_nonfinite_pixels_and_pulses(dev, oracle_lib):
+@pytest.mark.parametrize("N", [32, 64])
+def test_nonfinite_pixels_and_pulses(dev, oracle_lib, N):
     """NaN pixel positions contribute nothing and backproject to 0; pulses with NaN
-    geometry are skipped (their phase-history rows are 0), as the oracle does."""
+    geometry are skipped (their phase-history rows are 0), as the oracle does.
+    N = 32 runs the rocFFT stages, N = 64 the fused ones (whose row spans leave
+    a skipped pulse's row untouched); the forward is repeated after the adjoint
+    has written the rows."""
     from sarsolver_amd import SarPlan, synthetic
-    ap = synthetic.circular_aperture(16, 32, 2.0)
+    ap = synthetic.circular_aperture(16, N, 2.0)
     sc = synthetic.grid_scene(16, 0.25)
     pos = sc.positions.copy()
     pos[5] = np.nan
@@ -125,19 +129,21 @@ def test_nonfinite_pixels_and_pulses(dev, oracle_lib):
     tx[3, 1] = np.nan
     rng = np.random.default_rng(3)
     a = rng.standard_normal(len(pos)) + 1j * rng.standard_normal(len(pos))
-    y = rng.standard_normal((16, 32)) + 1j * rng.standard_normal((16, 32))
-    W, w = np.ones(32, complex), np.ones(16, complex)
+    y = rng.standard_normal((16, N)) + 1j * rng.standard_normal((16, N))
+    W, w = np.ones(N, complex), np.ones(16, complex)
     for precision in ("fp64", "series"):
         pl = SarPlan(tx, tx, ap.stab_ref_posns, pos, W, w, ap.centre_frequency,
                      ap.sample_bandwidth, ap.c_eff, 8.0, 1.0, interp="linear",
                      precision=precision, device=dev)
+        assert pl.layout["fused_fft_ratio"] == (8 if N == 64 else 0)
         Y = pl.forward(to_dev(a, dev)).cpu().numpy()
         X = pl.adjoint(to_dev(y, dev)).cpu().numpy()
+        Y2 = pl.forward(to_dev(a, dev)).cpu().numpy()
         Yo = oracle_lib.forward(tx, tx, ap.stab_ref_posns, pos, a, W, w, ap.centre_frequency,
-                                ap.sample_bandwidth, 32, upsample_ratio=8.0, interp="linear")
+                                ap.sample_bandwidth, N, upsample_ratio=8.0, interp="linear")
         Xo = oracle_lib.adjoint(tx, tx, ap.stab_ref_posns, pos, y, W, w, ap.centre_frequency,
                                 ap.sample_bandwidth, upsample_ratio=8.0, interp="linear")
-        assert np.all(Y[3] == 0) and X[5] == 0
+        assert np.all(Y[3] == 0) and X[5] == 0 and np.array_equal(Y, Y2)
         assert rel_l2(Y, Yo) <= 1e-5 and rel_l2(X, Xo) <= 1e-5
 
 
