@@ -273,6 +273,67 @@ __device__ __forceinline__ void eval_range(const KernelParams &k, const PulseDat
     }
 }
 
+// Series ranges with the bin scale folded into the polynomial, for kernels
+// that amortise per-(tile, pulse) constants over many pixels (the scatter's
+// lanes each own a pulse): per leg k_i = ku * c_i with c = (1/2, -1/8, 1/16,
+// -5/128, 7/256) the coefficients of g(t) = sqrt(1 + t) - 1, so
+// u = cu + sum_i k_i t^i by Horner's rule, and theta = rho u + phi with
+// rho = kc_rev / by_dx (u and theta are both dR times a constant) and
+// phi = ct - rho cu.  One FP64 instruction fewer per leg than forming g(t)
+// and then u and theta from it.
+template <bool MONO, int SORD>
+struct SeriesHorner {
+    double kt[SORD];             // leg T (monostatic: the doubled leg)
+    double kr[MONO ? 1 : SORD];  // leg R
+    double cu, phi;
+};
+
+template <bool MONO, int SORD>
+__device__ __forceinline__ SeriesHorner<MONO, SORD> make_horner(const PulseData<MONO> &d,
+                                                                const double rho) {
+    constexpr double c[5] = {0.5, -0.125, 0.0625, -0.0390625, 0.02734375};
+    SeriesHorner<MONO, SORD> h;
+#pragma unroll
+    for (int i = 0; i < SORD; ++i) {
+        if constexpr (MONO) {
+            h.kt[i] = d.s.ku * c[i];
+        } else {
+            h.kt[i] = d.s.kut * c[i];
+            h.kr[i] = d.s.kur * c[i];
+        }
+    }
+    h.cu = d.s.cu;
+    h.phi = fma(-rho, d.s.cu, d.s.ct);
+    return h;
+}
+
+// add + sum_i k_i t^(i+1), i < SORD
+template <int SORD>
+__device__ __forceinline__ double horner(const double t, const double (&k)[SORD], const double add) {
+    double v = k[SORD - 1];
+#pragma unroll
+    for (int i = SORD - 2; i >= 0; --i) v = fma(t, v, k[i]);
+    return fma(t, v, add);
+}
+
+template <bool MONO, bool PLANAR, int SORD>
+__device__ __forceinline__ void eval_range_horner(const PulseData<MONO> &d,
+                                                  const SeriesHorner<MONO, SORD> &h, const Pix &q,
+                                                  const double rho, double &u, double &th) {
+    if constexpr (MONO) {
+        const double t = PLANAR ? fma(d.s.bx, q.dx, fma(d.s.by, q.dy, d.s.c * q.d2))
+                                : fma(d.s.bx, q.dx, fma(d.s.by, q.dy, fma(d.s.bz, q.dz, d.s.c * q.d2)));
+        u = horner<SORD>(t, h.kt, h.cu);
+    } else {
+        const double zt = PLANAR ? d.s.tc * q.d2 : fma(d.s.tbz, q.dz, d.s.tc * q.d2);
+        const double zr = PLANAR ? d.s.rc * q.d2 : fma(d.s.rbz, q.dz, d.s.rc * q.d2);
+        const double tt = fma(d.s.tbx, q.dx, fma(d.s.tby, q.dy, zt));
+        const double tr = fma(d.s.rbx, q.dx, fma(d.s.rby, q.dy, zr));
+        u = horner<SORD>(tr, h.kr, horner<SORD>(tt, h.kt, h.cu));
+    }
+    th = fma(rho, u, h.phi);
+}
+
 // Nearest-neighbour bin (unwrapped, integer-valued double) with the
 // reference's rule; the series path re-evaluates exactly near a tie.
 template <int INTERP, int PREC, bool MONO>
@@ -623,6 +684,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const int t1 = min(k.num_tiles, t0 + R);
 
     const PolyRegs pr = make_poly_regs<SORD>();
+    const double rho = k.kc_rev / k.by_dx;  // revolutions per bin
     // region L1 norm (region_l1_kernel) -> fixed-point scale
     const float l1 = k.region_l1[region];
     const bool fixed_ok = isfinite(l1) && win > 0;
@@ -666,6 +728,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             PulseData<MONO> d;
             load_pulse<PREC, MONO>(k, tile, j, p, d);
             if constexpr (kRel) d.s.cu -= wb;
+            SeriesHorner<MONO, SORD> hc;
+            if constexpr (PREC == kPrecSeries) hc = make_horner<MONO, SORD>(d, rho);
             const int tb = tile * kTilePixels;
             const int tn = min(kTilePixels, k.M - tb);
             // NT == 512: the next pixel's record is requested one iteration ahead.
@@ -689,7 +753,10 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             // one pixel's contribution to this lane's pulse window
             auto contribute = [&](const Pix &px, const float2 a) {
                 double u, th;
-                eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
+                if constexpr (PREC == kPrecSeries)
+                    eval_range_horner<MONO, PLANAR, SORD>(d, hc, px, rho, u, th);
+                else
+                    eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
                 float sn, cs;
                 double fl;
                 if constexpr (INTERP == kInterpLinear && kRel) {
