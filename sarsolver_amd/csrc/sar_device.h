@@ -58,6 +58,15 @@ struct SeriesBi {
     double kut, kur, cu, ktt, ktr, ct, pad0, pad1;
 };
 static_assert(sizeof(SeriesMono) == 64, "SeriesMono must be 64 bytes");
+// Backprojection record of one (tile, pulse) for monostatic planar tiles with
+// the third-order series: t = bx dx + by dy + c |d|^2, u = cu + t (k1 + t (k2 +
+// t k3)) with k_i = ku (1/2, -1/8, 1/16) and cu moved by a multiple of nw into
+// the padded row (as the linear gather needs), theta = rho u + phi with
+// rho = kc_rev / by_dx and phi = ct - rho cu.
+struct SeriesGather {
+    double bx, by, c, k1, k2, k3, cu, phi;
+};
+static_assert(sizeof(SeriesGather) == 64, "SeriesGather must be 64 bytes");
 static_assert(sizeof(SeriesBi) == 128, "SeriesBi must be 128 bytes");
 
 // ||a - x|| with the reference's evaluation order: ((dx^2 + dy^2) + dz^2), sqrt.

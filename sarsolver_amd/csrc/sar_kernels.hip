@@ -404,20 +404,34 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
     // plan) so that every pixel of the tile lands in [2, nw + 2 shift_lo - 2):
     // bins are unchanged modulo nw and the padded row tail makes bins
     // n + 1 <= ld - 1 valid without a wrap.
+    // kHG: one SeriesGather record per (tile, pulse) replaces the series
+    // constants and the row shift (Horner form, phase from u).
+    constexpr bool kHG = kShift && MONO && PLANAR && PREC == kPrecSeries && SORD == 3;
+    const double rho = k.kc_rev / k.by_dx;
     int p_next = 0;
     PulseData<MONO> d_next;
+    SeriesGather g_next;
     if (j_begin < j_end) {
         p_next = k.rows[j_begin];
-        load_pulse<PREC, MONO>(k, tile, j_begin, p_next, d_next);
-        if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j_begin];
+        if constexpr (kHG) {
+            g_next = k.series_g[(size_t)tile * k.Pv + j_begin];
+        } else {
+            load_pulse<PREC, MONO>(k, tile, j_begin, p_next, d_next);
+            if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j_begin];
+        }
     }
     for (int j = j_begin; j < j_end; ++j) {
         const int p = p_next;
         PulseData<MONO> d = d_next;
+        const SeriesGather g = g_next;
         if (j + 1 < j_end) {
             p_next = k.rows[j + 1];
-            load_pulse<PREC, MONO>(k, tile, j + 1, p_next, d_next);
-            if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j + 1];
+            if constexpr (kHG) {
+                g_next = k.series_g[(size_t)tile * k.Pv + j + 1];
+            } else {
+                load_pulse<PREC, MONO>(k, tile, j + 1, p_next, d_next);
+                if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j + 1];
+            }
         }
         const float2 *row = k.prof + (size_t)p * k.ld;
         // 16-byte loads of bins (n, n + 1) through a buffer resource over the
@@ -432,7 +446,13 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
 #pragma unroll
             for (int m = 0; m < kPixPerThread; ++m) {
                 double u, th;
-                eval_range<PREC, MONO, PLANAR, SORD>(k, d, q[m], u, th, pr);
+                if constexpr (kHG) {
+                    const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, g.c * q[m].d2));
+                    u = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
+                    th = fma(rho, u, g.phi);
+                } else {
+                    eval_range<PREC, MONO, PLANAR, SORD>(k, d, q[m], u, th, pr);
+                }
                 // u > 0: truncation is floor, v_fract_f64 the exact rest
                 const int n0 = (int)u;
                 fr[m] = (float)__builtin_amdgcn_fract(u);
@@ -1139,6 +1159,32 @@ hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, c
                                 hipStream_t s) {
     if (Pv <= 0) return hipSuccess;
     adjoint_span_kernel<<<Pv, 256, 0, s>>>(pc, rows, tiles, nt, mono ? 1 : 0, by_dx, nw, span);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__restrict__ series,
+                                                            size_t n, double shift_lo,
+                                                            double inv_nw, double nwd, double rho,
+                                                            SeriesGather *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const SeriesMono s = series[i];
+    SeriesGather g;
+    g.bx = s.bx;
+    g.by = s.by;
+    g.c = s.c;
+    g.k1 = s.ku * 0.5;
+    g.k2 = s.ku * -0.125;
+    g.k3 = s.ku * 0.0625;
+    g.cu = fma(-floor((s.cu - shift_lo) * inv_nw), nwd, s.cu);  // as bp_shift_kernel
+    g.phi = fma(-rho, g.cu, s.ct);
+    out[i] = g;
+}
+
+hipError_t launch_series_gather(const void *series, size_t n, double shift_lo, double inv_nw,
+                                double nwd, double rho, SeriesGather *out, hipStream_t s) {
+    series_gather_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
+        (const SeriesMono *)series, n, shift_lo, inv_nw, nwd, rho, out);
     return hipGetLastError();
 }
 

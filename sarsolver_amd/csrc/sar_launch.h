@@ -50,6 +50,7 @@ struct KernelParams {
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
     const double *bp_cu;      // [num_tiles][Pv] series `cu` moved into [shift_lo, shift_lo + nw)
+    const SeriesGather *series_g;  // [num_tiles][Pv] gather records (monostatic planar, order 3) or NULL
     int series_order;         // series order of the linear gather / scatter (3 or 5)
     // variant
     int interp, precision;
@@ -102,6 +103,9 @@ hipError_t launch_region_l1(const KernelParams &k, hipStream_t s);  // before la
 // backprojection can read (span[p] for p = rows[j]; other entries untouched).
 hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, const double4 *tiles,
                                 int nt, bool mono, double by_dx, int nw, int2 *span, hipStream_t s);
+// Gather records from the monostatic series constants (cu moved as launch_bp_shift).
+hipError_t launch_series_gather(const void *series, size_t n, double shift_lo, double inv_nw,
+                                double nwd, double rho, SeriesGather *out, hipStream_t s);
 hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
                            double nwd, double *out, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)

@@ -130,6 +130,7 @@ struct sarhip_plan_s {
     bool fft_sparse = false;  // adjoint spans cover under half of the valid rows' bins
     float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
+    SeriesGather *d_series_g = nullptr;  // [num_tiles][Pv] gather records (see SeriesGather)
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
@@ -168,6 +169,7 @@ struct sarhip_plan_s {
         dev_free(d_pixrec);
         dev_free(d_series);
         dev_free(d_bp_cu);
+        dev_free(d_series_g);
         dev_free(d_srec);
         dev_free(d_fft_tables);
         dev_free(d_W);
@@ -208,6 +210,7 @@ struct sarhip_plan_s {
         k.Pv = Pv;
         k.series = d_series;
         k.bp_cu = d_bp_cu;
+        k.series_g = d_series_g;
         k.series_order = series_order;
         k.prof = d_prof;
         k.nw = (int)nw;
@@ -827,9 +830,18 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                                      pl->by_dx, pl->kc_rev, mono, pl->d_series, nullptr));
         if (interp == SARHIP_INTERP_LINEAR) {
             const size_t n = (size_t)pl->num_tiles * (size_t)pl->Pv;
-            HIP_TRY(hipMalloc((void **)&pl->d_bp_cu, sizeof(double) * n));
-            HIP_TRY(launch_bp_shift(pl->d_series, mono, n, pl->shift_lo, pl->inv_nw, (double)nw,
-                                    pl->d_bp_cu, nullptr));
+            // the monostatic planar third-order gather reads SeriesGather
+            // records (the kernel that selects it: bp_dispatch_prec)
+            if (mono && pl->planar && pl->precision == SARHIP_PRECISION_SERIES &&
+                pl->series_order == 3) {
+                HIP_TRY(hipMalloc((void **)&pl->d_series_g, sizeof(SeriesGather) * n));
+                HIP_TRY(launch_series_gather(pl->d_series, n, pl->shift_lo, pl->inv_nw, (double)nw,
+                                             pl->kc_rev / pl->by_dx, pl->d_series_g, nullptr));
+            } else {
+                HIP_TRY(hipMalloc((void **)&pl->d_bp_cu, sizeof(double) * n));
+                HIP_TRY(launch_bp_shift(pl->d_series, mono, n, pl->shift_lo, pl->inv_nw,
+                                        (double)nw, pl->d_bp_cu, nullptr));
+            }
         }
     }
 
