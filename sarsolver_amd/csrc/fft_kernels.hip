@@ -345,7 +345,8 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
 // Forward: Y[p][dst(k)] = W[k] wfac[p] sum_n prof[p][n] e^{-2 pi i s_k n / N_w}
 // (row stride nw = U N), k over the N band bins.  With `spans`, row p holds
 // data only in the spans[p].y bins from spans[p].x on (wrapping at N_w); the
-// other bins are taken as zero without being read.
+// other bins are taken as zero without being read; a count of -1 marks a
+// poisoned row, whose output is NaN.
 template <int N, int Q>
 __global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__restrict__ prof,
                                                                 const int2 *__restrict__ spans,
@@ -364,6 +365,10 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__
     const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
     const float2 *row = prof + (size_t)p * nw;
     const int2 sp = spans ? spans[p] : make_int2(0, nw);
+    if (sp.y < 0) {  // poisoned row (see the merge)
+        for (int k = tid; k < N; k += Bt::NT) Y[(size_t)p * N + k] = make_float2(NAN, NAN);
+        return;
+    }
     float2 acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc[q] = make_float2(0.f, 0.f);

@@ -804,21 +804,31 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             add_fix(ws + 2 * l + 2, r1, i1);
                         }
                     } else if (qok) {
-                        const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
-                        const int n0 = bin_index(k, kRel ? floor(u) + wb : fl, 1);
-                        const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
-                        atomicAdd(&row[n0].x, gr * cr);
-                        atomicAdd(&row[n0].y, gr * ci);
-                        atomicAdd(&row[n1].x, fu * cr);
-                        atomicAdd(&row[n1].y, fu * ci);
+                        if (k.fwd_spans) {
+                            // rows are not cleared and the merge stores spans:
+                            // the row is poisoned instead (NaN output row)
+                            atomicOr(&k.fpoison[p], 1);
+                        } else {
+                            const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
+                            const int n0 = bin_index(k, kRel ? floor(u) + wb : fl, 1);
+                            const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
+                            atomicAdd(&row[n0].x, gr * cr);
+                            atomicAdd(&row[n0].y, gr * ci);
+                            atomicAdd(&row[n1].x, fu * cr);
+                            atomicAdd(&row[n1].y, fu * ci);
+                        }
                     }
                 } else {
                     if (fixed_ok && (unsigned)l < (unsigned)win) {
                         if (qok) add_fix(ws + 2 * l, fix_tz(cr), fix_tz(ci));
                     } else if (qok) {
-                        const int nb = bin_index(k, fl, 1);
-                        atomicAdd(&row[nb].x, cr * unscale);
-                        atomicAdd(&row[nb].y, ci * unscale);
+                        if (k.fwd_spans) {
+                            atomicOr(&k.fpoison[p], 1);
+                        } else {
+                            const int nb = bin_index(k, fl, 1);
+                            atomicAdd(&row[nb].x, cr * unscale);
+                            atomicAdd(&row[nb].y, ci * unscale);
+                        }
                     }
                 }
             };
@@ -940,6 +950,17 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     const float *wm = k.wmax + (size_t)j * NR;
     const float2 *part = k.partial + (size_t)j * NR * win;
     const int tid = (int)threadIdx.x;
+    // a row the scatter poisoned: no span, NaN output row (flag cleared here)
+    __shared__ int s_poison;
+    if (tid == 0) s_poison = k.fwd_spans ? k.fpoison[p] : 0;
+    __syncthreads();
+    if (s_poison) {
+        if (tid == 0) {
+            k.fspan[p] = make_int2(0, -1);
+            k.fpoison[p] = 0;
+        }
+        return;
+    }
     double lo = INFINITY, hi = -INFINITY;
     float l1 = 0.f;
     for (int t = tid; t < NR; t += kRedThreads) {
@@ -967,10 +988,13 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     const bool fixed = isfinite(l1);
     const float scale = (fixed && l1 > 0.f) ? 2147483648.0f / (l1 * 1.001f) : 0.f;
     const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
-    // The covered bins [lo, hi) are written (stored) once each, and their span
-    // is recorded for the spectral stage; the rest of the row is not touched.
-    // A span longer than the row wraps onto itself: the row is zeroed first
-    // and the chunks add into it.
+    // With fwd_spans the covered bins [lo, hi) are written (stored) once each,
+    // and their span is recorded for the spectral stage; the rest of the row
+    // is not touched (a span longer than the row wraps onto itself: the row is
+    // zeroed first and the chunks add into it).  Otherwise the row was cleared
+    // before the scatter, which may have added global atomics, and the chunks
+    // add into it.
+    const bool store = k.fwd_spans != 0;
     const bool empty = !(hi > lo) || (fixed && scale == 0.f);  // every window zero
     const bool full = !empty && hi - lo > (double)k.nw;
     if (tid == 0)
@@ -980,7 +1004,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     int *acc_re = (int *)acc, *acc_im = acc_re + chunk;
     float *facc_re = (float *)acc_re, *facc_im = (float *)acc_im;
     float2 *row = k.prof + (size_t)p * k.nw;
-    if (full) {
+    if (full && store) {
         for (int e = tid; e < k.nw; e += kRedThreads) row[e] = make_float2(0.f, 0.f);
         __threadfence_block();
         __syncthreads();
@@ -1034,7 +1058,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
         const int len = (int)fmin((double)chunk, hi - c0);
         for (int e = tid; e < len; e += kRedThreads) {
             const int n = bin_index(k, c0 + (double)e, 1);
-            float2 r = full ? row[n] : make_float2(0.f, 0.f);
+            float2 r = (full || !store) ? row[n] : make_float2(0.f, 0.f);
             if (fixed) {
                 r.x += (float)acc_re[e] * inv_scale;
                 r.y += (float)acc_im[e] * inv_scale;

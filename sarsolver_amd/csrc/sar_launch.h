@@ -45,7 +45,10 @@ struct KernelParams {
     float2 *partial;          // [Pv][num_regions][win]
     double *wbases;           // [Pv][num_regions]
     float *wmax;              // [Pv][num_regions] max |component| of each window
-    int2 *fspan;              // [P] covered bins of each forward row: {first bin, count}
+    int2 *fspan;              // [P] covered bins of each forward row: {first bin, count}; count
+                              // -1: the row is poisoned (a contribution could not be windowed)
+    int *fpoison;             // [P] set by the scatter, read and cleared by the merge
+    int fwd_spans;            // 1: the merge stores spans (fused spectral stage, rows not cleared)
     float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1

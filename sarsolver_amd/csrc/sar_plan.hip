@@ -139,6 +139,7 @@ struct sarhip_plan_s {
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
     int2 *d_fspan = nullptr;  // [P] forward row spans written by the merge
+    int *d_fpoison = nullptr;  // [P] rows the scatter could not window (cleared by the merge)
     int2 *d_aspan = nullptr;  // [P] adjoint row spans the backprojection reads (fused path)
     float *d_region_l1 = nullptr;
     double2 *d_bp_partial = nullptr;
@@ -182,6 +183,7 @@ struct sarhip_plan_s {
         dev_free(d_wbases);
         dev_free(d_wmax);
         dev_free(d_fspan);
+        dev_free(d_fpoison);
         dev_free(d_aspan);
         dev_free(d_region_l1);
         dev_free(d_bp_partial);
@@ -234,6 +236,8 @@ struct sarhip_plan_s {
         k.wbases = d_wbases;
         k.wmax = d_wmax;
         k.fspan = d_fspan;
+        k.fpoison = d_fpoison;
+        k.fwd_spans = 0;
         k.region_l1 = d_region_l1;
         k.bp_splits = bp_splits;
         k.bp_partial = d_bp_partial;
@@ -803,6 +807,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     // rows the merge never writes (invalid pulses) keep the empty span
     HIP_TRY(hipMalloc((void **)&pl->d_fspan, sizeof(int2) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMemset(pl->d_fspan, 0, sizeof(int2) * std::max<size_t>(P, 1)));
+    HIP_TRY(hipMalloc((void **)&pl->d_fpoison, sizeof(int) * std::max<size_t>(P, 1)));
+    HIP_TRY(hipMemset(pl->d_fpoison, 0, sizeof(int) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
     // The planar 64-pulse linear scatter streams 32-byte records that the
     // region-norm kernel writes on every forward call.
@@ -1130,7 +1136,8 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
     if (pl->Pv > 0) {
         HIP_TRY(launch_gather_perm((const float2 *)amps, pl->d_perm, (int)pl->M, pl->d_amps_perm,
                                    s));
-        const KernelParams k = pl->params();
+        KernelParams k = pl->params();
+        k.fwd_spans = spans ? 1 : 0;
         HIP_TRY(launch_region_l1(k, s));
         if ((rc = record(pl, 4, s))) return rc;
         HIP_TRY(launch_fp_bin(k, s));

@@ -163,3 +163,39 @@ def test_dense_scene_forward_fixed_point(dev, oracle_lib, interp):
     Yo = oracle_forward(oracle_lib, ap, sc.positions, x, w.upsample, interp)
     print(f"dense {interp}: forward {rel_l2(Y, Yo):.2e}")
     assert rel_l2(Y, Yo) <= 1e-5
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_nonfinite_amplitudes_poison_the_forward(dev, oracle_lib, fused):
+    """A non-finite amplitude makes every row of the reference's forward
+    non-finite (its FFT spreads the value over the row).  The scatter cannot
+    window such a region: with the fused spectral stage (row spans, rows not
+    cleared) it poisons the row and the output row is NaN; with rocFFT it adds
+    global atomics into the cleared rows.  Either way every valid row comes
+    out non-finite, and the next call with finite amplitudes is exact again
+    (the poison flags are cleared)."""
+    import os
+    from sarsolver_amd import SarPlan, synthetic
+    ap = synthetic.circular_aperture(16, 64, 2.0)
+    sc = synthetic.grid_scene(64, 0.25)
+    rng = np.random.default_rng(8)
+    a = rng.standard_normal(sc.num_scatterers) + 1j * rng.standard_normal(sc.num_scatterers)
+    W, w = np.ones(64, complex), np.ones(16, complex)
+    os.environ["SARHIP_FUSED_FFT"] = "1" if fused else "0"
+    try:
+        pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
+                     sc.positions, W, w, ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, 8.0,
+                     1.0, interp="linear", precision="series", device=dev)
+    finally:
+        os.environ.pop("SARHIP_FUSED_FFT", None)
+    assert pl.layout["fused_fft_ratio"] == (8 if fused else 0) and pl.layout["fwd_window_bins"] > 0
+    print(f"case: non-finite amplitudes fused={fused}", flush=True)
+    bad = a.copy()
+    bad[100] = np.nan
+    Yb = pl.forward(to_dev(bad, dev)).cpu().numpy()
+    assert not np.any(np.isfinite(Yb).all(axis=1))
+    Y = pl.forward(to_dev(a, dev)).cpu().numpy()
+    Yo = oracle_lib.forward(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns,
+                            sc.positions, a, W, w, ap.centre_frequency, ap.sample_bandwidth, 64,
+                            upsample_ratio=8.0, interp="linear")
+    assert rel_l2(Y, Yo) <= 1e-5
