@@ -287,9 +287,13 @@ int record(sarhip_plan_t pl, int i, hipStream_t s) {
 // Forward LDS window length (bins) for regions with bounding boxes `reg` /
 // half-extents `half`: per leg |T - X| - |T - Xc| lies in
 // [-s, s + r^2 / (2 (A - r))], s = sum_i |e_i| h_i, so the bins of one
-// (region, pulse) span at most (2 S + Q) by_dx plus rounding and the linear
-// neighbour; the window keeps a margin of 8 bins.  Returns INFINITY if a leg
-// is inside a region's bounding sphere.
+// (region, pulse) span at most y = (2 S + Q) by_dx.  The scatter's window
+// base sits 2 bins below the lowest bin (floored), so a pixel's relative u
+// lies in [2, y + 3 + e), e the range rounding (far below one bin in every
+// precision mode); the linear rule touches floor(u) + 1 <= ceil(y) + 4 and
+// the nearest rules round(u + 1/2) <= ceil(y) + 4, hence ceil(y) + 5 bins.
+// A contribution outside its window poisons the row (it is never dropped).
+// Returns INFINITY if a leg is inside a region's bounding sphere.
 double window_bins(const std::vector<double4> &reg, const std::vector<double4> &half,
                    const std::vector<PulseConst> &pc, const std::vector<int> &rows, bool mono,
                    double by_dx) {
@@ -315,7 +319,7 @@ double window_bins(const std::vector<double4> &reg, const std::vector<double4> &
             worst = std::max(worst, (2.0 * S + Q) * by_dx);
         }
     }
-    return std::ceil(worst * (1.0 + 1e-9)) + 8.0;
+    return std::ceil(worst * (1.0 + 1e-9)) + 5.0;
 }
 
 // Largest truncation term of the 5th-order series, 21/1024 t^6 per leg, in
@@ -615,6 +619,29 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     for (int k = 0; k < 3; ++k)
         if (hi[k] > lo[k]) span = std::max(span, hi[k] - lo[k]);
     const double qscale = span > 0.0 ? (double)((1 << 21) - 1) / span : 0.0;
+    // Morton axis order: the lowest bits go to the axis along which the
+    // platform is seen least (mean |e_k| over valid pulses and legs, e the
+    // unit vector from the scene centre to the phase centre).  Tiles stay
+    // compact either way; pairs of consecutive tiles (2-tile forward
+    // regions) then extend across range, where a region's projected range
+    // extent, and so its LDS window, grows least.
+    int axes[3] = {0, 1, 2};
+    {
+        double m[3] = {0.0, 0.0, 0.0};
+        const double ctr[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+        if (pl->Pv > 0 && std::isfinite(ctr[0] + ctr[1] + ctr[2])) {
+            for (int p : rows) {
+                for (int leg = 0; leg < 2; ++leg) {
+                    const double *T = leg == 0 ? pc[p].tx : pc[p].rx;
+                    const double d[3] = {T[0] - ctr[0], T[1] - ctr[1], T[2] - ctr[2]};
+                    const double A = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                    if (A > 0.0)
+                        for (int k = 0; k < 3; ++k) m[k] += std::fabs(d[k]) / A;
+                }
+            }
+            std::stable_sort(axes, axes + 3, [&](int a, int b) { return m[a] < m[b]; });
+        }
+    }
     {
         DevBuf<unsigned long long> keys_in, keys_out;
         DevBuf<int> idx_in;
@@ -622,7 +649,7 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         HIP_TRY(keys_out.alloc(M));
         HIP_TRY(idx_in.alloc(M));
         HIP_TRY(hipMalloc((void **)&pl->d_perm, sizeof(int) * M));
-        HIP_TRY(launch_morton(d_pos, (int)M, lo, qscale, keys_in.p, idx_in.p, nullptr));
+        HIP_TRY(launch_morton(d_pos, (int)M, lo, qscale, axes, keys_in.p, idx_in.p, nullptr));
         size_t temp_bytes = 0;
         HIP_TRY(sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, idx_in.p, pl->d_perm,
                            (int)M, nullptr));

@@ -136,7 +136,7 @@ __device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
 // truncated; non-finite pixels get the largest key (they sort last).
 __global__ __launch_bounds__(kBlock) void morton_kernel(const double *__restrict__ pos, int M,
                                                         double lo0, double lo1, double lo2,
-                                                        double qscale,
+                                                        double qscale, int a0, int a1, int a2,
                                                         unsigned long long *__restrict__ keys,
                                                         int *__restrict__ idx) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const double *__restrict
             t = fmin(fmax(t, 0.0), (double)((1 << 21) - 1));
             q[k] = (unsigned long long)t;
         }
-        code = spread21(q[0]) | (spread21(q[1]) << 1) | (spread21(q[2]) << 2);
+        code = spread21(q[a0]) | (spread21(q[a1]) << 1) | (spread21(q[a2]) << 2);
     }
     keys[m] = code;
     idx[m] = m;
@@ -296,8 +296,9 @@ hipError_t launch_bbox(const double *pos, int M, double *part, int part_blocks, 
 }
 
 hipError_t launch_morton(const double *pos, int M, const double lo[3], double qscale,
-                         unsigned long long *keys, int *idx, hipStream_t s) {
-    morton_kernel<<<blocks_for(M), kBlock, 0, s>>>(pos, M, lo[0], lo[1], lo[2], qscale, keys, idx);
+                         const int axes[3], unsigned long long *keys, int *idx, hipStream_t s) {
+    morton_kernel<<<blocks_for(M), kBlock, 0, s>>>(pos, M, lo[0], lo[1], lo[2], qscale, axes[0],
+                                                   axes[1], axes[2], keys, idx);
     return hipGetLastError();
 }
 
