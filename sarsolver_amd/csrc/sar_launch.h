@@ -74,9 +74,11 @@ struct GridSpec {
 hipError_t launch_grid_positions(const GridSpec &g, double *pos, hipStream_t s);
 hipError_t launch_bbox(const double *pos, int M, double *part, int part_blocks, double *out,
                        hipStream_t s);
-// axes: the coordinate of each Morton bit position (axes[0] takes the lowest bits)
+// scale: per-coordinate factor on the quantisation; axes: the coordinate of
+// each Morton bit position (axes[0] takes the lowest bits)
 hipError_t launch_morton(const double *pos, int M, const double lo[3], double qscale,
-                         const int axes[3], unsigned long long *keys, int *idx, hipStream_t s);
+                         const double scale[3], const int axes[3], unsigned long long *keys,
+                         int *idx, hipStream_t s);
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const unsigned long long *keys_in,
                       unsigned long long *keys_out, const int *vals_in, int *vals_out, int M,
                       hipStream_t s);

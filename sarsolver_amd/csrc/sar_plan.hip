@@ -328,11 +328,14 @@ double window_bins(const std::vector<double4> &reg, const std::vector<double4> &
 // (tile, pulse): |t| <= (2 A r + r^2) / A^2 for a tile of radius r at range A,
 // and the remainder is at most the first omitted term / (1 - |t|) (the terms
 // alternate for t > 0 and shrink geometrically for t < 0).
-double series_error_bound(const std::vector<double4> &tiles, const std::vector<PulseConst> &pc,
-                          const std::vector<int> &rows, bool mono, double by_dx, int order) {
+double series_error_bound(const std::vector<double4> &tiles, const std::vector<double4> &half,
+                          const std::vector<PulseConst> &pc, const std::vector<int> &rows,
+                          bool mono, double by_dx, int order) {
     const double coef = order == 3 ? 5.0 / 128.0 : 21.0 / 1024.0;
     double worst = 0.0;
-    for (const double4 &tl : tiles) {
+    for (size_t i = 0; i < tiles.size(); ++i) {
+        const double4 &tl = tiles[i];
+        const double4 &h = half[i];
         const double r = tl.w;
         for (int p : rows) {
             double legs = 0.0;
@@ -341,7 +344,10 @@ double series_error_bound(const std::vector<double4> &tiles, const std::vector<P
                 const double dx = T[0] - tl.x, dy = T[1] - tl.y, dz = T[2] - tl.z;
                 const double A = std::sqrt(dx * dx + dy * dy + dz * dz);
                 if (!(A > 2.0 * r)) return INFINITY;
-                const double t = (2.0 * A * r + r * r) / (A * A);
+                // |b.d| = 2 |a.d| / A^2 <= 2 sum_i |a_i| h_i / A^2 (|d_i| <= h_i) and
+                // c |d|^2 <= r^2 / A^2: the tile's extent projected on the leg
+                const double s = std::fabs(dx) * h.x + std::fabs(dy) * h.y + std::fabs(dz) * h.z;
+                const double t = (2.0 * s + r * r) / (A * A);
                 if (!(t < 0.5)) return INFINITY;
                 legs += coef * std::pow(t, order + 1) / (1.0 - t) * A * (mono ? 2.0 : 1.0);
             }
@@ -355,6 +361,12 @@ int env_int(const char *name, int dflt) {
     const char *v = std::getenv(name);
     if (!v || !*v) return dflt;
     return std::atoi(v);
+}
+
+double env_double(const char *name, double dflt) {
+    const char *v = std::getenv(name);
+    if (!v || !*v) return dflt;
+    return std::atof(v);
 }
 
 }  // namespace
@@ -626,6 +638,7 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     // regions) then extend across range, where a region's projected range
     // extent, and so its LDS window, grows least.
     int axes[3] = {0, 1, 2};
+    double qs_axis[3] = {1.0, 1.0, 1.0};
     {
         double m[3] = {0.0, 0.0, 0.0};
         const double ctr[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
@@ -640,6 +653,12 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                 }
             }
             std::stable_sort(axes, axes + 3, [&](int a, int b) { return m[a] < m[b]; });
+            // strongly directional looks: the cross-range axis is quantised 4x
+            // coarser, so tiles are 4x longer across range than along it
+            // (C2: 16 x 64 pixels; windows of 2-tile regions 63 bins, against
+            // 101 for square tiles)
+            const double f = env_double("SARHIP_TILE_ASPECT", 0.25);
+            if (m[axes[0]] <= 0.25 * m[axes[2]] && f > 0.0 && f <= 1.0) qs_axis[axes[0]] = f;
         }
     }
     {
@@ -649,7 +668,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         HIP_TRY(keys_out.alloc(M));
         HIP_TRY(idx_in.alloc(M));
         HIP_TRY(hipMalloc((void **)&pl->d_perm, sizeof(int) * M));
-        HIP_TRY(launch_morton(d_pos, (int)M, lo, qscale, axes, keys_in.p, idx_in.p, nullptr));
+        HIP_TRY(launch_morton(d_pos, (int)M, lo, qscale, qs_axis, axes, keys_in.p, idx_in.p,
+                              nullptr));
         size_t temp_bytes = 0;
         HIP_TRY(sort_pairs(nullptr, temp_bytes, keys_in.p, keys_out.p, idx_in.p, pl->d_perm,
                            (int)M, nullptr));
@@ -723,12 +743,12 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         auto ok = [&](double bins) {
             return bins <= 1e-9 && std::fabs(bins * pl->kc_rev / pl->by_dx) <= 1e-9;
         };
-        pl->series_err_bins = series_error_bound(tiles, pc, rows, mono, pl->by_dx, 5);
+        pl->series_err_bins = series_error_bound(tiles, tile_half, pc, rows, mono, pl->by_dx, 5);
         if (precision == SARHIP_PRECISION_SERIES && !ok(pl->series_err_bins))
             pl->precision = SARHIP_PRECISION_FP64;
         if (pl->precision == SARHIP_PRECISION_SERIES && interp == SARHIP_INTERP_LINEAR &&
             env_int("SARHIP_SERIES_ORDER", 3) == 3) {
-            const double e3 = series_error_bound(tiles, pc, rows, mono, pl->by_dx, 3);
+            const double e3 = series_error_bound(tiles, tile_half, pc, rows, mono, pl->by_dx, 3);
             if (ok(e3)) {
                 pl->series_order = 3;
                 pl->bp_series_err_bins = e3;
@@ -754,10 +774,12 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         double best_cost = INFINITY;
         int best_r = 1, best_g = 32, best_w = 0;
         // 64 pulses per workgroup (512 threads, wave-uniform pixel streams)
-        // whenever its windows fit in 64 KiB; otherwise 32 or 16 pulses under
-        // `budget`.  Among region sizes, fewest window bins per tile.
+        // whenever its windows fit in 53 KiB (three workgroups per CU; with
+        // two, C2's 8-tile regions ran slower than 2-tile ones in 32 KiB);
+        // otherwise 32 or 16 pulses under `budget`.  Among region sizes,
+        // fewest window bins per tile.
         const size_t budget64 =
-            (size_t)std::min(std::max(env_int("SARHIP_FP_LDS64_KB", 64), 16), 160) * 1024;
+            (size_t)std::min(std::max(env_int("SARHIP_FP_LDS64_KB", 53), 16), 160) * 1024;
         for (int pass = 0; pass < 2 && !std::isfinite(best_cost); ++pass) {
             for (int R : {1, 2, 4, 8, 16}) {
                 if (forced_r > 0 && R != forced_r) continue;

@@ -136,7 +136,8 @@ __device__ __forceinline__ unsigned long long spread21(unsigned long long v) {
 // truncated; non-finite pixels get the largest key (they sort last).
 __global__ __launch_bounds__(kBlock) void morton_kernel(const double *__restrict__ pos, int M,
                                                         double lo0, double lo1, double lo2,
-                                                        double qscale, int a0, int a1, int a2,
+                                                        double qscale, double f0, double f1,
+                                                        double f2, int a0, int a1, int a2,
                                                         unsigned long long *__restrict__ keys,
                                                         int *__restrict__ idx) {
     const int m = blockIdx.x * kBlock + threadIdx.x;
@@ -146,8 +147,9 @@ __global__ __launch_bounds__(kBlock) void morton_kernel(const double *__restrict
     unsigned long long code = ~0ULL;
     if (finite3(v[0], v[1], v[2])) {
         unsigned long long q[3];
+        const double f[3] = {f0, f1, f2};
         for (int k = 0; k < 3; ++k) {
-            double t = (v[k] - lo[k]) * qscale;
+            double t = (v[k] - lo[k]) * qscale * f[k];
             t = fmin(fmax(t, 0.0), (double)((1 << 21) - 1));
             q[k] = (unsigned long long)t;
         }
@@ -296,9 +298,11 @@ hipError_t launch_bbox(const double *pos, int M, double *part, int part_blocks, 
 }
 
 hipError_t launch_morton(const double *pos, int M, const double lo[3], double qscale,
-                         const int axes[3], unsigned long long *keys, int *idx, hipStream_t s) {
-    morton_kernel<<<blocks_for(M), kBlock, 0, s>>>(pos, M, lo[0], lo[1], lo[2], qscale, axes[0],
-                                                   axes[1], axes[2], keys, idx);
+                         const double scale[3], const int axes[3], unsigned long long *keys,
+                         int *idx, hipStream_t s) {
+    morton_kernel<<<blocks_for(M), kBlock, 0, s>>>(pos, M, lo[0], lo[1], lo[2], qscale, scale[0],
+                                                   scale[1], scale[2], axes[0], axes[1], axes[2],
+                                                   keys, idx);
     return hipGetLastError();
 }
 
