@@ -783,6 +783,13 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         for (int pass = 0; pass < 2 && !std::isfinite(best_cost); ++pass) {
             for (int R : {1, 2, 4, 8, 16}) {
                 if (forced_r > 0 && R != forced_r) continue;
+                // The fixed-point unit is (region L1 norm) / 2^31, so the
+                // windows' rounding grows with the region's pixel count
+                // (dense-image forward against the FP64 oracle: 2.2e-7 with
+                // float atomics, 5e-7 / 9e-7 / 1.8e-6 / 3.6e-6 for regions of
+                // 1 / 2 / 4 / 8 tiles); regions stay within 2 tiles unless
+                // forced.
+                if (forced_r == 0 && R > 2) continue;
                 std::vector<double4> reg, hal;
                 if ((rc = tile_stats((size_t)R * kTilePixels, false, reg, hal))) return rc;
                 const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
