@@ -98,8 +98,6 @@ struct PosSource {
     GridSpec grid{};
 };
 
-constexpr size_t kLdsBudget = 64 * 1024;
-
 }  // namespace
 
 struct sarhip_plan_s {
