@@ -552,13 +552,15 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 // every sum stays inside int32; FP32 products carry ~6e-8 relative error and
 // the truncation adds less than 2^-31 of the norm per contribution.  The
 // integer parts of the two linear-interpolation shares sum to the truncated
-// total.  A region whose amplitudes are not finite uses float global atomics.
+// total.  A region whose amplitudes are not finite (and any contribution
+// outside its window, which the window bounds rule out) cannot be windowed:
+// with cleared rows it goes to float global atomics, on the span path the
+// row is poisoned instead.
 //
 // Windows are converted back to float and written whole (zeros included) to
 // `partial` with their base bins and max |component|, then merged per pulse
 // by fp_reduce_kernel: no global atomics on the main path and a result that
-// does not depend on scheduling.  A contribution outside its window (not
-// expected) falls back to a global atomic into the profile.
+// does not depend on scheduling.
 // ---------------------------------------------------------------------------
 // Toward zero (one v_cvt_i32_f32).  Its error is odd in v, so bin sums of
 // contributions with mixed signs carry no bias; rounding halves always up
@@ -931,8 +933,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
 // 2^31 / (1.001 x the sum over windows of the window's max |component|), a
 // bound on any bin sum of the row (rounding adds at most half a unit per
 // window, far inside the 0.1% margin); if that bound is not finite the chunk
-// accumulates in float.
-// The chunk is then written into the row.
+// accumulates in float.  The chunk is then stored into the row (span path)
+// or added to it (rows cleared before the scatter).
 constexpr int kRedThreads = 512;
 constexpr int kRedWin = 4;     // windows per wave per batch
 constexpr int kRedSlices = 2;  // 64-bin slices per window per batch
