@@ -870,11 +870,14 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         (pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO))
         HIP_TRY(hipMalloc((void **)&pl->d_srec, sizeof(ScatRec) * (M + 4)));  // + group overrun
     // Backprojection: split each tile's pulses over several workgroups until
-    // the grid has about 8 workgroups per CU (occupancy), keeping at least 128
-    // pulses per split; partial images are summed in FP64 in split order.
+    // the grid has about 32 workgroups per CU, keeping at least 128 pulses per
+    // split; partial images are summed in FP64 in split order.  (C2, 1024
+    // tiles: 2 / 4 / 6 / 8 / 12 / 16 splits took 1.64 / 1.56 / 1.54 / 1.49 /
+    // 1.47 / 1.48 ms; the smaller counts leave the last wave of workgroups
+    // short of work.)
     {
         int S = 1;
-        while ((long long)pl->num_tiles * S < 2048 && pl->Pv / (2 * S) >= 128) S *= 2;
+        while ((long long)pl->num_tiles * S < 8192 && pl->Pv / (2 * S) >= 128) S *= 2;
         const int forced = env_int("SARHIP_BP_SPLITS", 0);
         if (forced > 0) S = std::min(forced, std::max(pl->Pv, 1));
         pl->bp_splits = S;
