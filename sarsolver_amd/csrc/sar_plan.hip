@@ -17,6 +17,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sarsolver_hip.h"
@@ -282,6 +283,31 @@ int record(sarhip_plan_t pl, int i, hipStream_t s) {
     return SARHIP_OK;
 }
 
+// max over i < n of f(i) (>= 0; a non-finite value gives INFINITY), on up to
+// 16 host threads; max is exact, so the result does not depend on the split.
+template <class F>
+double parallel_max(size_t n, F f) {
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t T = std::min(hw, std::max<size_t>(1, n / 8));
+    std::vector<double> part(T, 0.0);
+    auto run = [&](size_t t) {
+        double w = 0.0;
+        for (size_t i = t; i < n; i += T) {
+            const double v = f(i);
+            w = std::isfinite(v) ? std::max(w, v) : INFINITY;
+            if (!std::isfinite(w)) break;
+        }
+        part[t] = w;
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < T; ++t) pool.emplace_back(run, t);
+    run(0);
+    for (std::thread &th : pool) th.join();
+    double w = 0.0;
+    for (const double v : part) w = std::max(w, v);
+    return w;
+}
+
 // Forward LDS window length (bins) for regions with bounding boxes `reg` /
 // half-extents `half`: per leg |T - X| - |T - Xc| lies in
 // [-s, s + r^2 / (2 (A - r))], s = sum_i |e_i| h_i, so the bins of one
@@ -295,17 +321,17 @@ int record(sarhip_plan_t pl, int i, hipStream_t s) {
 double window_bins(const std::vector<double4> &reg, const std::vector<double4> &half,
                    const std::vector<PulseConst> &pc, const std::vector<int> &rows, bool mono,
                    double by_dx) {
-    double worst = 0.0;
-    for (size_t t = 0; t < reg.size(); ++t) {
+    const double worst = parallel_max(reg.size(), [&](size_t t) {
         const double4 &c = reg[t];
         const double4 &h = half[t];
+        double w = 0.0;
         for (int p : rows) {
             double S = 0.0, Q = 0.0;
             for (int leg = 0; leg < 2; ++leg) {
                 const double *T = leg == 0 ? pc[p].tx : pc[p].rx;
                 const double ax = T[0] - c.x, ay = T[1] - c.y, az = T[2] - c.z;
                 const double A = std::sqrt(ax * ax + ay * ay + az * az);
-                if (!(A > c.w)) return INFINITY;
+                if (!(A > c.w)) return (double)INFINITY;
                 S += (std::fabs(ax) * h.x + std::fabs(ay) * h.y + std::fabs(az) * h.z) / A;
                 Q += c.w * c.w / (2.0 * (A - c.w));
                 if (mono) {
@@ -314,45 +340,45 @@ double window_bins(const std::vector<double4> &reg, const std::vector<double4> &
                     break;
                 }
             }
-            worst = std::max(worst, (2.0 * S + Q) * by_dx);
+            w = std::max(w, (2.0 * S + Q) * by_dx);
         }
-    }
+        return w;
+    });
     return std::ceil(worst * (1.0 + 1e-9)) + 5.0;
 }
 
-// Largest truncation term of the 5th-order series, 21/1024 t^6 per leg, in
-// bins, over every (tile, valid pulse); t <= (2 A r + r^2) / A^2.
 // Bound (bins) on the truncation of sqrt(1 + t) - 1 after t^order over every
-// (tile, pulse): |t| <= (2 A r + r^2) / A^2 for a tile of radius r at range A,
-// and the remainder is at most the first omitted term / (1 - |t|) (the terms
+// (tile, pulse): |t| <= (2 s + r^2) / A^2 for a tile of radius r and extent s
+// projected on the leg at range A, and the remainder is at most the first
+// omitted term / (1 - |t|) (the terms
 // alternate for t > 0 and shrink geometrically for t < 0).
 double series_error_bound(const std::vector<double4> &tiles, const std::vector<double4> &half,
                           const std::vector<PulseConst> &pc, const std::vector<int> &rows,
                           bool mono, double by_dx, int order) {
     const double coef = order == 3 ? 5.0 / 128.0 : 21.0 / 1024.0;
-    double worst = 0.0;
-    for (size_t i = 0; i < tiles.size(); ++i) {
+    return parallel_max(tiles.size(), [&](size_t i) {
         const double4 &tl = tiles[i];
         const double4 &h = half[i];
         const double r = tl.w;
+        double w = 0.0;
         for (int p : rows) {
             double legs = 0.0;
             for (int leg = 0; leg < (mono ? 1 : 2); ++leg) {
                 const double *T = leg == 0 ? pc[p].tx : pc[p].rx;
                 const double dx = T[0] - tl.x, dy = T[1] - tl.y, dz = T[2] - tl.z;
                 const double A = std::sqrt(dx * dx + dy * dy + dz * dz);
-                if (!(A > 2.0 * r)) return INFINITY;
+                if (!(A > 2.0 * r)) return (double)INFINITY;
                 // |b.d| = 2 |a.d| / A^2 <= 2 sum_i |a_i| h_i / A^2 (|d_i| <= h_i) and
                 // c |d|^2 <= r^2 / A^2: the tile's extent projected on the leg
                 const double s = std::fabs(dx) * h.x + std::fabs(dy) * h.y + std::fabs(dz) * h.z;
                 const double t = (2.0 * s + r * r) / (A * A);
-                if (!(t < 0.5)) return INFINITY;
+                if (!(t < 0.5)) return (double)INFINITY;
                 legs += coef * std::pow(t, order + 1) / (1.0 - t) * A * (mono ? 2.0 : 1.0);
             }
-            worst = std::max(worst, legs * by_dx);
+            w = std::max(w, legs * by_dx);
         }
-    }
-    return worst;
+        return w;
+    });
 }
 
 int env_int(const char *name, int dflt) {
