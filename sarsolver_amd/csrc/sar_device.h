@@ -59,12 +59,13 @@ struct SeriesBi {
 };
 static_assert(sizeof(SeriesMono) == 64, "SeriesMono must be 64 bytes");
 // Backprojection record of one (tile, pulse) for monostatic planar tiles with
-// the third-order series: t = bx dx + by dy + c |d|^2, u = cu + t (k1 + t (k2 +
-// t k3)) with k_i = ku (1/2, -1/8, 1/16) and cu moved by a multiple of nw into
-// the padded row (as the linear gather needs), theta = rho u + phi with
-// rho = kc_rev / by_dx and phi = ct - rho cu.
+// the third-order series, the argument scaled by 1/c so that it takes two
+// FMAs: s = t / c = bx' dx + by' dy + |d|^2 with bx' = bx / c, by' = by / c,
+// u = cu + s (k1 + s (k2 + s k3)) with k_i = ku (1/2, -1/8, 1/16)_i c^i and cu
+// moved by a multiple of nw into the padded row (as the linear gather needs),
+// theta = rho u + phi with rho = kc_rev / by_dx and phi = ct - rho cu.
 struct SeriesGather {
-    double bx, by, c, k1, k2, k3, cu, phi;
+    double bx, by, k1, k2, k3, cu, phi, pad;
 };
 static_assert(sizeof(SeriesGather) == 64, "SeriesGather must be 64 bytes");
 static_assert(sizeof(SeriesBi) == 128, "SeriesBi must be 128 bytes");

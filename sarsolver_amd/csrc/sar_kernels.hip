@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
             for (int m = 0; m < kPixPerThread; ++m) {
                 double u, th;
                 if constexpr (kHG) {
-                    const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, g.c * q[m].d2));
+                    const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, q[m].d2));
                     u = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
                     th = fma(rho, u, g.phi);
                 } else {
@@ -1196,12 +1196,13 @@ __global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__
     if (i >= n) return;
     const SeriesMono s = series[i];
     SeriesGather g;
-    g.bx = s.bx;
-    g.by = s.by;
-    g.c = s.c;
-    g.k1 = s.ku * 0.5;
-    g.k2 = s.ku * -0.125;
-    g.k3 = s.ku * 0.0625;
+    const double inv_c = 1.0 / s.c, c2 = s.c * s.c;
+    g.bx = s.bx * inv_c;
+    g.by = s.by * inv_c;
+    g.k1 = s.ku * 0.5 * s.c;
+    g.k2 = s.ku * -0.125 * c2;
+    g.k3 = s.ku * 0.0625 * (c2 * s.c);
+    g.pad = 0.0;
     g.cu = fma(-floor((s.cu - shift_lo) * inv_nw), nwd, s.cu);  // as bp_shift_kernel
     g.phi = fma(-rho, g.cu, s.ct);
     out[i] = g;
