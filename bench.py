@@ -191,8 +191,9 @@ def pcg_comparison(dev, scenes: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU "
-              "sharding is a design in DESIGN.md and is not run here.", file=sys.stderr)
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}): sharding the "
+              "pulses or pixels over several GPUs is designed but not built or run.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from sarsolver_amd import SarPlan, synthetic
