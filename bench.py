@@ -11,14 +11,17 @@ Also reported:
   roofline      the dominant kernel (the longer of the forward scatter and the
                 backprojection gather) against the MI355X FP32 vector peak:
                 algorithmic flop per pixel-pulse (40 scatter / 39 gather,
-                linear interpolation) x M x P per launch / mean launch time,
-                measured with HIP events around that launch on the library's
-                stream; the other kernel as roofline_other_kernel;
+                linear interpolation, 8 less on monostatic pulses) x M x P per
+                launch / mean launch time, measured with HIP events around that
+                launch on the library's stream; the other kernel as
+                roofline_other_kernel;
+  spectral_stage  each operator's spectral stage against HBM: 8 P (N + N_w)
+                bytes per application / its mean device time;
   cpu_baseline  the FP64 CPU oracle (a restatement of the reference algorithm:
                 per-pulse scatter/gather + full N_w FFT per scene block, threads
-                over scene blocks as the reference's joblib pool) on a bounded
-                sample of the same workload (a pulse subset), scaled per
-                pixel-pulse;
+                over scene blocks as the reference's joblib pool) on bounded
+                samples (pulse subsets of this workload on all cores and on one
+                core, and the whole 128^2 workload), medians of 5 runs;
   pcg           time to relative residual 1e-3 for (A^H A + 1e-3 I) x = A^H y
                 (A normalised by its 2-norm) on held-out 20 dB scenes of the
                 512^2 workload (1024 pulses x 2048 samples, 3 deg): plain CGLS,
@@ -41,13 +44,16 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# Algorithmic flop per pixel-pulse of the two main kernels (series ranges):
-# range 20 (t: 3 mul + 2 add; fifth-order series 5 fma + 1 mul; u and theta
-# 2 fma), sin + cos 2; gather: linear interpolation 4 + complex multiply-
-# accumulate 8 + bin/fraction 5 (nearest: 2 + 8); scatter: complex product 6,
-# interpolation shares 4, accumulation 4, bin/fraction 4 (nearest: 6 + 2).
+# Algorithmic flop per pixel-pulse of the two main kernels, as the hot-path
+# work model counts them: two distances 18 (3 sub + 3 mul + 2 add + 1 sqrt
+# each), minus the SRP range 2, bin scale and fraction 2, complex lerp 6,
+# phase and sin/cos 3, complex multiply 6, accumulate 2 = 39 for the linear
+# backprojection; 32 without interpolation (nearest rules); 40 / 32 for the
+# forward.  A monostatic pulse evaluates one distance, counted twice above:
+# 8 less (the plan reports `monostatic`).
 FLOP_PER_PX_PULSE = {"linear": 39.0, "nearest": 32.0, "nearest_ref": 32.0}
-FWD_FLOP_PER_PX_PULSE = {"linear": 40.0, "nearest": 30.0, "nearest_ref": 30.0}
+FWD_FLOP_PER_PX_PULSE = {"linear": 40.0, "nearest": 32.0, "nearest_ref": 32.0}
+MONOSTATIC_SAVING = 8.0
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
@@ -60,7 +66,7 @@ def parse():
     ap.add_argument("--workload", default="c2_large")
     ap.add_argument("--interp", default="linear")
     ap.add_argument("--precision", default="series")
-    ap.add_argument("--cpu-sample-pulses", type=int, default=512)
+    ap.add_argument("--cpu-sample-pulses", type=int, default=128)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy a 512 MiB buffer once before timing (HBM counter scaling)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -71,13 +77,9 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(w, ap, pos, x, y, interp, pulses):
-    """Time the FP64 oracle (T = cores - 1 scene-block threads) on the first `pulses`
-    pulses of the workload, forward + adjoint; returns pixel-pulse/s."""
-    import oracle.oracle as O
-    O.build()
-    ncores = len(os.sched_getaffinity(0))
-    threads = max(1, ncores - 1)
+def _oracle_pass(O, ap, pos, x, y, U, interp, pulses, threads):
+    """One forward + one adjoint of the FP64 oracle on the first `pulses` pulses
+    (all pixels); returns wall seconds."""
     sl = slice(0, pulses)
     tx = ap.transmitter_positions[sl]
     srp = ap.stab_ref_posns[sl]
@@ -86,20 +88,65 @@ def cpu_baseline(w, ap, pos, x, y, interp, pulses):
     args = (ap.centre_frequency, ap.sample_bandwidth)
     t0 = time.perf_counter()
     O.forward(tx, tx, srp, pos, x, W, wts, *args, ap.num_freqs, c_eff=ap.c_eff,
-              upsample_ratio=w.upsample, interp=interp, num_threads=threads)
+              upsample_ratio=U, interp=interp, num_threads=threads)
     O.adjoint(tx, tx, srp, pos, np.fft.ifftshift(y[sl], axes=1), W, wts, *args, c_eff=ap.c_eff,
-              upsample_ratio=w.upsample, interp=interp, num_threads=threads)
-    dt = time.perf_counter() - t0
-    pp = 2.0 * pos.shape[0] * pulses
-    return {"value": pp / dt / 1e9, "unit": "Gpixel-pulse/s", "cores": threads, "kind": "port",
-            "sample": f"{w.name} geometry, first {pulses} of {w.pulses} pulses, all "
-                      f"{pos.shape[0]} pixels, one forward + one adjoint ({interp}, FP64, "
-                      f"own FFT), {dt:.1f} s wall"}
+              upsample_ratio=U, interp=interp, num_threads=threads)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(w, ap, pos, x, y, interp, pulses, repeats=5):
+    """The FP64 CPU oracle (a restatement of the reference algorithm: per-pulse
+    scatter / gather + a full N_w FFT per pulse in every scene block, T = cores
+    - 1 scene-block threads as the reference's joblib pool) on bounded samples,
+    each the median of `repeats` runs of one forward + one adjoint:
+      value        the benchmarked workload, first `pulses` pulses, all pixels, T threads;
+      single_thread  the same workload, first pulses // 8 pulses, 1 thread (per-core rate);
+      c1_small     the whole 128^2 / 256 x 512 workload, T threads.
+    Rates in Gpixel-pulse/s (2 M P' / median wall)."""
+    import ctypes.util
+    import oracle.oracle as O
+    from sarsolver_amd import synthetic
+    O.build()
+    ncores = len(os.sched_getaffinity(0))
+    threads = max(1, ncores - 1)
+    fft = ("FFTW present on the host but not used: the oracle's own radix-2 / Bluestein FFT"
+           if ctypes.util.find_library("fftw3") else
+           "the oracle's own radix-2 / Bluestein FFT (no FFTW on the host)")
+
+    def med(args, n):
+        ts = [_oracle_pass(O, *args) for _ in range(n)]
+        return float(np.median(ts)), ts
+
+    t_main, ts_main = med((ap, pos, x, y, w.upsample, interp, pulses, threads), repeats)
+    p1 = max(1, pulses // 8)
+    t_one, _ = med((ap, pos, x, y, w.upsample, interp, p1, 1), repeats)
+    w1 = synthetic.WORKLOADS["c1_small"]
+    ap1, sc1 = synthetic.build(w1)
+    g = np.random.default_rng(w1.seed)
+    x1 = synthetic.scene_amplitudes(w1, g).reshape(-1)
+    y1 = synthetic.complex_normal(g, (w1.pulses, w1.samples))
+    t_c1, _ = med((ap1, sc1.positions, x1, y1, w1.upsample, interp, w1.pulses, threads), repeats)
+    m = pos.shape[0]
+    rate = lambda mm, pp, t: 2.0 * mm * pp / t / 1e9
+    return {"value": rate(m, pulses, t_main), "unit": "Gpixel-pulse/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{w.name} geometry, first {pulses} of {w.pulses} pulses, all {m} pixels, "
+                      f"one forward + one adjoint ({interp}, FP64), median of {repeats} runs "
+                      f"({', '.join(f'{t:.2f}' for t in ts_main)} s) on {threads} threads "
+                      f"(host has {ncores}); FFT: {fft}",
+            "single_thread": {"value": rate(m, p1, t_one), "cores": 1,
+                              "sample": f"{w.name}, first {p1} pulses, median of {repeats}"},
+            "c1_small": {"value": rate(sc1.num_scatterers, w1.pulses, t_c1), "cores": threads,
+                         "sample": f"whole c1_small workload ({w1.n}^2 px, {w1.pulses} x "
+                                   f"{w1.samples}), median of {repeats}, "
+                                   f"{t_c1 * 1e3:.1f} ms per forward + adjoint"}}
 
 
 def traffic_from_profile(kernel: str):
-    """HBM bytes per launch of `kernel`, measured with rocprofv3 --pmc FETCH_SIZE /
-    WRITE_SIZE and committed under profiles/ (null if not measured)."""
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE runs of this benchmark (profiles/traffic.json):
+    static, measured on the build the file names, not in this run (null if
+    absent)."""
     f = ROOT / "profiles" / "traffic.json"
     if not f.exists():
         return None
@@ -196,7 +243,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
-    from sarsolver_amd import SarPlan, synthetic
+    from sarsolver_amd import synthetic
 
     w = synthetic.WORKLOADS[args.workload]
     rng = np.random.default_rng(w.seed)
@@ -204,18 +251,14 @@ def main():
     pos = sc.positions
     x = synthetic.scene_amplitudes(w, rng).reshape(-1)
     dev = torch.device("cuda", 0)
-    pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
-                 np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting, ap.centre_frequency,
-                 ap.sample_bandwidth, ap.c_eff, w.upsample, ap.sign_multiplier,
-                 interp=args.interp, precision=args.precision, device=dev)
+    pl = synthetic.workload_plan(w, ap, pos, dev, interp=args.interp, precision=args.precision)
     xd = torch.from_numpy(x.astype(np.complex64)).to(dev)
     # The timed forward runs on a dense image (complex Gaussian, every pixel
     # non-zero), as the iterates of a solver are: the forward skips regions of
     # the image that are exactly zero, so the sparse scene would credit it
     # with pixel-pulses it never processed.  The scene's own rate is reported
     # separately.
-    g = np.random.default_rng(w.seed + 7)
-    xdense = (g.standard_normal(pos.shape[0]) + 1j * g.standard_normal(pos.shape[0])) / np.sqrt(2)
+    xdense = synthetic.dense_image(w, pos.shape[0])
     xdd = torch.from_numpy(xdense.astype(np.complex64)).to(dev)
     if args.calibrate_copy:
         src = torch.ones(512 * 2 ** 20 // 4, dtype=torch.float32, device=dev)
@@ -255,9 +298,11 @@ def main():
     value = 2.0 * pp / (ms_step * 1e-3) / 1e9
     # roofline of the dominant kernel (longer mean launch, HIP events around
     # the launch on the library stream); the other main kernel alongside
+    mono_save = MONOSTATIC_SAVING if pl.layout["monostatic"] else 0.0
     kern = {
-        "bp_gather_kernel": (FLOP_PER_PX_PULSE[args.interp], t_adj["main_kernel"]),
-        "fp_scatter_kernel": (FWD_FLOP_PER_PX_PULSE[args.interp], t_fwd["main_kernel"]),
+        "bp_gather_kernel": (FLOP_PER_PX_PULSE[args.interp] - mono_save, t_adj["main_kernel"]),
+        "fp_scatter_kernel": (FWD_FLOP_PER_PX_PULSE[args.interp] - mono_save,
+                              t_fwd["main_kernel"]),
     }
     def roof(name):
         fpp, ms = kern[name]
@@ -265,8 +310,22 @@ def main():
         return {"bound": "valu", "kernel": name, "achieved": round(achieved, 3),
                 "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
-                "traffic": traffic_from_profile(name), "launch_ms": round(ms, 4),
+                "traffic": traffic_from_profile(name),
+                "traffic_source": "static: profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE / "
+                                  "WRITE_SIZE of this benchmark, scaled by a copy kernel)",
+                "launch_ms": round(ms, 4),
+                "launch_ms_source": "mean of HIP events around each launch on the library "
+                                    "stream over the timed steps",
                 "algorithmic_flop_per_px_pulse": fpp}
+    # spectral stage of each operator against HBM: 8 P (N + N_w) bytes per
+    # application (read the band, write the upsampled rows, or the reverse)
+    nw = int(w.upsample * w.samples)
+    spec_bytes = 8.0 * P * (w.samples + nw)
+    def spectral(t):
+        gbs = spec_bytes / (t["fft"] * 1e-3) / 1e9
+        return {"bound": "hbm", "algorithmic_bytes": spec_bytes, "ms": round(t["fft"], 4),
+                "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
     kernel = max(kern, key=lambda n: kern[n][1])
     other = [n for n in kern if n != kernel][0]
     line = {
@@ -294,6 +353,7 @@ def main():
                      "forward": {k: round(v, 4) for k, v in t_fwd.items()}},
         "roofline": roof(kernel),
         "roofline_other_kernel": roof(other),
+        "spectral_stage": {"adjoint": spectral(t_adj), "forward": spectral(t_fwd)},
         "wall_s_timed": round(wall, 3),
         "plan_layout": pl.layout,
     }

@@ -92,10 +92,15 @@ __global__ __launch_bounds__(256) void truncate_kernel(const float2 *__restrict_
     Y[(size_t)p * N + dst] = v;
 }
 
-__global__ __launch_bounds__(256) void fill_zero_kernel(float4 *__restrict__ buf, size_t n4) {
+// Zero `bytes` bytes at a 16-byte aligned `buf`: 16-byte stores, then the
+// remaining bytes % 16 (e.g. the last complex64 of an odd element count).
+__global__ __launch_bounds__(256) void fill_zero_kernel(float4 *__restrict__ buf, size_t bytes) {
+    const size_t n4 = bytes / 16;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t stride = (size_t)gridDim.x * 256;
     for (; i < n4; i += stride) buf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (blockIdx.x == 0 && threadIdx.x < bytes - 16 * n4)
+        ((unsigned char *)(buf + n4))[threadIdx.x] = 0;
 }
 
 // amps_perm[i] = amps[perm[i]]: pixel amplitudes into tile (Morton) order.
@@ -568,11 +573,33 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 // values keep only a few fractional bits.  At the 2^31 scale the truncation
 // error matches that of round-to-nearest at 2^30.
 __device__ __forceinline__ int fix_tz(const float v) { return (int)v; }
-// Fixed-point scale of a region with L1 norm l1: 2^31 / (1.001 l1), so that no
-// bin sum of the region reaches 2^31 / 1.001 (interpolation weights are <= 1).
-__device__ __forceinline__ float fixed_scale(const float l1) {
-    return 2147483648.0f / (l1 * 1.001f);
+// Fixed-point scale 2^31 / (1.001 b) for a bound b on every bin sum, held as
+// m 2^e with m in (1, 2]: values are scaled by ldexp(v m, e) and sums taken
+// back by ldexp(s / m, -e).  2^31 / b itself leaves the FP32 range for b below
+// about 6e-30 (and 1 / that scale for b below 1e-29 is subnormal); the split
+// form stays exact there, so the forward stays linear for tiny inputs.
+struct FixScale {
+    float m, inv_m;
+    int e;
+};
+// b finite and > 0 (and 1.001 b finite).
+__device__ __forceinline__ FixScale fix_scale(const float b) {
+    int ex;
+    const float mm = frexpf(b * 1.001f, &ex);  // 1.001 b = mm 2^ex, mm in [1/2, 1)
+    FixScale f;
+    f.m = 1.0f / mm;
+    f.inv_m = mm;
+    f.e = 31 - ex;
+    return f;
 }
+__device__ __forceinline__ float fix_apply(const FixScale &f, const float v) {
+    return ldexpf(v * f.m, f.e);
+}
+__device__ __forceinline__ float fix_undo(const FixScale &f, const float v) {
+    return ldexpf(v * f.inv_m, -f.e);
+}
+// Whether a bound admits fixed-point windows: finite with 1.001 b finite.
+__device__ __forceinline__ bool fix_bound_ok(const float b) { return isfinite(b * 1.001f); }
 // Read-only element at a wave-uniform index, through the constant address
 // space so that the compiler may use scalar loads.
 typedef const __attribute__((address_space(4))) double *const_f64_ptr;
@@ -621,39 +648,142 @@ __device__ __forceinline__ void add_fix(unsigned int *bin, const int re, const i
     atomicAdd(bin + 1, (unsigned)im);
 }
 
-// Per-region L1 norm sum(|Re a| + |Im a|) of the permuted amplitudes (FP32,
-// fixed reduction order), and the amplitudes scaled in place by the region's
-// fixed-point scale (when windows are used and the norm is finite and
-// non-zero), so the scatter forms fixed-point contributions without a multiply.
+// Window base (bin) of forward region `region` for pulse row p: for a leg with
+// phase centre T, unit vector e from the region centre Xc to T and a pixel
+// X = Xc + d in the region's box of half-extents h, |T - X| - |T - Xc| lies in
+// [-s, s + r^2 / (2 (|T - Xc| - r))] with s = sum_i |e_i| h_i and r the region
+// radius, so dR - dR(Xc) >= -(s_T + s_R); the base sits two bins below.
+template <bool MONO>
+__device__ __forceinline__ double window_base(const KernelParams &k, const int region,
+                                              const int p) {
+    const double4 rc = k.region_c[region];
+    const double4 rh = k.region_h[region];
+    double sleg;
+    const PulseConst &c = k.pc[p];
+    const double ax = c.tx[0] - rc.x, ay = c.tx[1] - rc.y, az = c.tx[2] - rc.z;
+    const double inv = 1.0 / sqrt(fma(ax, ax, fma(ay, ay, az * az)));
+    sleg = (fabs(ax) * rh.x + fabs(ay) * rh.y + fabs(az) * rh.z) * inv;
+    if (!MONO) {
+        const double bx = c.rx[0] - rc.x, by = c.rx[1] - rc.y, bz = c.rx[2] - rc.z;
+        const double invr = 1.0 / sqrt(fma(bx, bx, fma(by, by, bz * bz)));
+        sleg += (fabs(bx) * rh.x + fabs(by) * rh.y + fabs(bz) * rh.z) * invr;
+    } else {
+        sleg *= 2.0;
+    }
+    return floor((ref_dr<MONO>(c, rc.x, rc.y, rc.z) - sleg * (1.0 + 1e-9) - 1e-9) * k.by_dx) - 2.0;
+}
+
+// Once per plan: for each forward region, the largest number of valid pixels
+// whose floor(u) is n - 1 or n, over the bins n and valid pulses (the count C
+// of region_l1_kernel's bound).  u is the exact FP64 range (the reference's
+// formula) relative to the scatter's window base; a pixel within 1e-6 bin of a
+// bin edge counts on both sides, which covers the series modes' deviation
+// from it (below 1e-9 bin).  One workgroup per (region, PB pulses), lane
+// q % PB owns a pulse's histogram in LDS; a u outside the window (which the
+// window bounds rule out) gives -1: no count bound for that region.
+// cmax[region] must start at 0.
+template <bool MONO>
+__global__ __launch_bounds__(256) void region_count_kernel(const KernelParams k, int PB,
+                                                           int *__restrict__ cmax) {
+    extern __shared__ int hist[];  // [PB][wst]
+    __shared__ int s_over;
+    const int region = blockIdx.x;
+    const int j0 = blockIdx.y * PB;
+    const int tid = (int)threadIdx.x;
+    const int q = tid % PB, stream = tid / PB, nstreams = 256 / PB;
+    const int np = min(PB, k.Pv - j0);
+    const bool qok = q < np;
+    const int p = k.rows[j0 + (qok ? q : np - 1)];
+    const int win = k.win, wst = k.win_stride;
+    if (tid == 0) s_over = 0;
+    for (int e = tid; e < PB * wst; e += 256) hist[e] = 0;
+    __syncthreads();
+    const double wb = window_base<MONO>(k, region, p);
+    int *h = hist + q * wst;
+    const int pb = region * k.region_tiles * kTilePixels;
+    const int pe = min(k.M, pb + k.region_tiles * kTilePixels);
+    bool over = false;
+    for (int i = pb + stream; i < pe; i += nstreams) {
+        const PixRec r = k.pixrec[i];
+        if (r.ok == 0.0 || !qok) continue;
+        const double u = ref_dr<MONO>(k.pc[p], r.x, r.y, r.z) * k.by_dx - wb;
+        const double a = floor(u - 1e-6), b = floor(u + 1e-6);
+        if (!(a >= 0.0) || !(b < (double)win)) {
+            over = true;
+            continue;
+        }
+        atomicAdd(&h[(int)a], 1);
+        if (b != a) atomicAdd(&h[(int)b], 1);
+    }
+    if (over) s_over = 1;
+    __syncthreads();
+    if (tid < np) {
+        int best = 0, prev = 0;
+        for (int n = 0; n < win; ++n) {
+            const int c = hist[tid * wst + n];
+            best = max(best, prev + c);
+            prev = c;
+        }
+        atomicMax(&cmax[region], s_over ? 0x7fffffff : best);
+    }
+}
+
+// Per-region bound on every window bin sum of the forward, and the amplitudes
+// scaled in place by the region's fixed-point scale (when windows are used and
+// the bound is finite and non-zero), so the scatter forms fixed-point
+// contributions without a multiply.
+//
+// A bin n of one pulse's window receives shares of weight <= 1 from the pixels
+// whose floor(u) is n - 1 or n, each of magnitude <= |Re a| + |Im a| per
+// component, so |sum| <= C a_max with C the largest number of such pixels over
+// the region's bins and pulses (`cmax`, counted once per plan by
+// region_count_kernel; NULL or negative: not counted) and a_max the largest
+// |Re a| + |Im a| of the region.  The L1 norm sum(|Re a| + |Im a|) is a bound
+// too; the smaller one is taken.  (For a 2048-pixel region C is about 64, and
+// C a_max is about a ninth of the L1 norm of a dense image: the fixed-point
+// unit, bound / 2^31, is as much finer.)  Sums in FP32 in a fixed order;
+// a non-finite L1 norm is kept (the scatter then takes no windows).
 __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict__ amps,
                                                              int M, int region_pixels,
                                                              int windows,
                                                              float *__restrict__ out,
+                                                             const int *__restrict__ cmax,
                                                              const PixRec *__restrict__ pixrec,
                                                              ScatRec *__restrict__ srec) {
-    __shared__ float red[kThreads];
+    __shared__ float red[kThreads], redm[kThreads];
     const int base = blockIdx.x * region_pixels;
     const int n = min(region_pixels, M - base);
-    float l1 = 0.f;
+    float l1 = 0.f, am = 0.f;
     for (int i = threadIdx.x; i < n; i += kThreads) {
         const float2 a = amps[base + i];
-        l1 += fabsf(a.x) + fabsf(a.y);
+        const float v = fabsf(a.x) + fabsf(a.y);
+        l1 += v;
+        am = fmaxf(am, v);
     }
     red[threadIdx.x] = l1;
+    redm[threadIdx.x] = am;
     __syncthreads();
     for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if ((int)threadIdx.x < s) {
+            red[threadIdx.x] += red[threadIdx.x + s];
+            redm[threadIdx.x] = fmaxf(redm[threadIdx.x], redm[threadIdx.x + s]);
+        }
         __syncthreads();
     }
-    const float tot = red[0];
-    if (threadIdx.x == 0) out[blockIdx.x] = tot;
-    const bool scale = windows && isfinite(tot) && tot > 0.f;
-    const float sc = scale ? fixed_scale(tot) : 1.f;
+    float bound = red[0];
+    if (cmax && isfinite(bound)) {
+        const int c = cmax[blockIdx.x];
+        if (c >= 0) bound = fminf(bound, (float)c * redm[0]);
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = bound;
+    const bool scale = windows && fix_bound_ok(bound) && bound > 0.f;
+    FixScale fs{1.f, 1.f, 0};
+    if (scale) fs = fix_scale(bound);
     if (scale || srec) {
         for (int i = threadIdx.x; i < n; i += kThreads) {
             float2 a = amps[base + i];
             if (scale) {
-                a = make_float2(a.x * sc, a.y * sc);
+                a = make_float2(fix_apply(fs, a.x), fix_apply(fs, a.y));
                 amps[base + i] = a;
             }
             if (srec) {
@@ -707,45 +837,25 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
 
     const PolyRegs pr = make_poly_regs<SORD>();
     const double rho = k.kc_rev / k.by_dx;  // revolutions per bin
-    // region L1 norm (region_l1_kernel) -> fixed-point scale
-    const float l1 = k.region_l1[region];
-    const bool fixed_ok = isfinite(l1) && win > 0;
-    const float scale = (fixed_ok && l1 > 0.f) ? fixed_scale(l1) : 0.f;
-    const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
-    // amplitudes arrive pre-scaled by `scale` when it is non-zero
-    const float unscale = scale > 0.f ? inv_scale : 1.0f;
+    // region bound (region_l1_kernel) -> fixed-point scale
+    const float bound = k.region_l1[region];
+    const bool fixed_ok = fix_bound_ok(bound) && win > 0;
+    const bool zero_region = fixed_ok && bound == 0.f;
+    // amplitudes arrive pre-scaled by `fs` when `scaled`
+    const bool scaled = fixed_ok && bound > 0.f;
+    FixScale fs{1.f, 1.f, 0};
+    if (scaled) fs = fix_scale(bound);
+    auto unscale = [&](const float v) { return scaled ? fix_undo(fs, v) : v; };
     // linear series / hilo: u is evaluated relative to the window base
     constexpr bool kRel = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
     constexpr bool kCompact = kRel && PLANAR && NT == 512;
-
-    // Window base: for a leg with phase centre T, unit vector e from the region
-    // centre Xc to T and a pixel X = Xc + d in the region's box of half-extents h,
-    // |T - X| - |T - Xc| lies in [-s, s + r^2 / (2 (|T - Xc| - r))] with
-    // s = sum_i |e_i| h_i and r the region radius, so dR - dR(Xc) >= -(s_T + s_R).
-    const double4 rc = k.region_c[region];
-    const double4 rh = k.region_h[region];
-    double sleg;
-    {
-        const PulseConst &c = k.pc[p];
-        const double ax = c.tx[0] - rc.x, ay = c.tx[1] - rc.y, az = c.tx[2] - rc.z;
-        const double inv = 1.0 / sqrt(fma(ax, ax, fma(ay, ay, az * az)));
-        sleg = (fabs(ax) * rh.x + fabs(ay) * rh.y + fabs(az) * rh.z) * inv;
-        if (!MONO) {
-            const double bx = c.rx[0] - rc.x, by = c.rx[1] - rc.y, bz = c.rx[2] - rc.z;
-            const double invr = 1.0 / sqrt(fma(bx, bx, fma(by, by, bz * bz)));
-            sleg += (fabs(bx) * rh.x + fabs(by) * rh.y + fabs(bz) * rh.z) * invr;
-        } else {
-            sleg *= 2.0;
-        }
-    }
-    const double wb =
-        floor((ref_dr<MONO>(k.pc[p], rc.x, rc.y, rc.z) - sleg * (1.0 + 1e-9) - 1e-9) * k.by_dx) - 2.0;
+    const double wb = window_base<MONO>(k, region, p);
     const int wst = k.win_stride;
     for (int e = tid; e < G * wst; e += NT) wsum[e] = 0ull;
     __syncthreads();
     unsigned int *ws = (unsigned int *)(wsum + q * wst);
     float2 *row = k.prof + (size_t)p * k.nw;
-    if (!(fixed_ok && scale == 0.f)) {  // skip all-zero regions
+    if (!zero_region) {  // skip all-zero regions
         for (int tile = t0; tile < t1; ++tile) {
             PulseData<MONO> d;
             load_pulse<PREC, MONO>(k, tile, j, p, d);
@@ -811,13 +921,13 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             // the row is poisoned instead (NaN output row)
                             atomicOr(&k.fpoison[p], 1);
                         } else {
-                            const float gr = (1.0f - fr) * unscale, fu = fr * unscale;
+                            const float gr = 1.0f - fr;
                             const int n0 = bin_index(k, kRel ? floor(u) + wb : fl, 1);
                             const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
-                            atomicAdd(&row[n0].x, gr * cr);
-                            atomicAdd(&row[n0].y, gr * ci);
-                            atomicAdd(&row[n1].x, fu * cr);
-                            atomicAdd(&row[n1].y, fu * ci);
+                            atomicAdd(&row[n0].x, unscale(gr * cr));
+                            atomicAdd(&row[n0].y, unscale(gr * ci));
+                            atomicAdd(&row[n1].x, unscale(fr * cr));
+                            atomicAdd(&row[n1].y, unscale(fr * ci));
                         }
                     }
                 } else {
@@ -828,8 +938,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             atomicOr(&k.fpoison[p], 1);
                         } else {
                             const int nb = bin_index(k, fl, 1);
-                            atomicAdd(&row[nb].x, cr * unscale);
-                            atomicAdd(&row[nb].y, ci * unscale);
+                            atomicAdd(&row[nb].x, unscale(cr));
+                            atomicAdd(&row[nb].y, unscale(ci));
                         }
                     }
                 }
@@ -913,7 +1023,8 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
         for (int l = lane; l < win; l += 64) {
             const unsigned long long w = wsum[qq * wst + l];
             const int2 v = make_int2((int)(unsigned)(w & 0xffffffffull), (int)(unsigned)(w >> 32));
-            const float2 f = make_float2((float)v.x * inv_scale, (float)v.y * inv_scale);
+            const float2 f = scaled ? make_float2(fix_undo(fs, (float)v.x), fix_undo(fs, (float)v.y))
+                                    : make_float2(0.f, 0.f);
             dst[l] = f;
             mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
         }
@@ -987,9 +1098,9 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     lo = red_lo[0];
     hi = red_hi[0] + (double)win;
     l1 = red_l1[0];
-    const bool fixed = isfinite(l1);
-    const float scale = (fixed && l1 > 0.f) ? 2147483648.0f / (l1 * 1.001f) : 0.f;
-    const float inv_scale = scale > 0.f ? 1.0f / scale : 0.f;
+    const bool fixed = fix_bound_ok(l1);
+    FixScale fs{1.f, 1.f, 0};
+    if (fixed && l1 > 0.f) fs = fix_scale(l1);
     // With fwd_spans the covered bins [lo, hi) are written (stored) once each,
     // and their span is recorded for the spectral stage; the rest of the row
     // is not touched (a span longer than the row wraps onto itself: the row is
@@ -997,7 +1108,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     // before the scatter, which may have added global atomics, and the chunks
     // add into it.
     const bool store = k.fwd_spans != 0;
-    const bool empty = !(hi > lo) || (fixed && scale == 0.f);  // every window zero
+    const bool empty = !(hi > lo) || (fixed && l1 == 0.f);  // every window zero
     const bool full = !empty && hi - lo > (double)k.nw;
     if (tid == 0)
         k.fspan[p] = empty ? make_int2(0, 0)
@@ -1045,8 +1156,8 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
                         if (l < win && rel >= 0.0 && rel < (double)chunk) {
                             const int r = (int)rel;
                             if (fixed) {
-                                atomicAdd(&acc_re[r], (int)rintf(v[w][sl].x * scale));
-                                atomicAdd(&acc_im[r], (int)rintf(v[w][sl].y * scale));
+                                atomicAdd(&acc_re[r], (int)rintf(fix_apply(fs, v[w][sl].x)));
+                                atomicAdd(&acc_im[r], (int)rintf(fix_apply(fs, v[w][sl].y)));
                             } else {
                                 atomicAdd(&facc_re[r], v[w][sl].x);
                                 atomicAdd(&facc_im[r], v[w][sl].y);
@@ -1062,8 +1173,8 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
             const int n = bin_index(k, c0 + (double)e, 1);
             float2 r = (full || !store) ? row[n] : make_float2(0.f, 0.f);
             if (fixed) {
-                r.x += (float)acc_re[e] * inv_scale;
-                r.y += (float)acc_im[e] * inv_scale;
+                r.x += fix_undo(fs, (float)acc_re[e]);
+                r.y += fix_undo(fs, (float)acc_im[e]);
             } else {
                 r.x += facc_re[e];
                 r.y += facc_im[e];
@@ -1105,7 +1216,7 @@ hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s) {
     size_t blocks = (n4 + 255) / 256;
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) blocks = 1;
-    fill_zero_kernel<<<(unsigned)blocks, 256, 0, s>>>((float4 *)buf, n4);
+    fill_zero_kernel<<<(unsigned)blocks, 256, 0, s>>>((float4 *)buf, bytes);
     return hipGetLastError();
 }
 
@@ -1311,8 +1422,21 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
 hipError_t launch_region_l1(const KernelParams &k, hipStream_t s) {
     region_l1_kernel<<<k.num_regions, kThreads, 0, s>>>(const_cast<float2 *>(k.amps_perm), k.M,
                                                         k.region_tiles * kTilePixels, k.win > 0,
-                                                        k.region_l1, k.pixrec,
+                                                        k.region_l1, k.region_cmax, k.pixrec,
                                                         const_cast<ScatRec *>(k.srec));
+    return hipGetLastError();
+}
+
+hipError_t launch_region_count(const KernelParams &k, int *cmax, hipStream_t s) {
+    if (k.win <= 0 || k.num_regions <= 0 || k.Pv <= 0) return hipSuccess;
+    // pulses per workgroup: histograms of 64, 32 or 16 pulses within 48 KiB
+    int PB = 64;
+    while (PB > 16 && (size_t)PB * k.win_stride * sizeof(int) > 48 * 1024) PB /= 2;
+    if ((size_t)PB * k.win_stride * sizeof(int) > 48 * 1024) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)k.num_regions, (unsigned)((k.Pv + PB - 1) / PB));
+    const size_t lds = (size_t)PB * k.win_stride * sizeof(int);
+    if (k.mono) region_count_kernel<true><<<grid, 256, lds, s>>>(k, PB, cmax);
+    else region_count_kernel<false><<<grid, 256, lds, s>>>(k, PB, cmax);
     return hipGetLastError();
 }
 

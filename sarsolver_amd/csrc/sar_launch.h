@@ -49,7 +49,9 @@ struct KernelParams {
                               // -1: the row is poisoned (a contribution could not be windowed)
     int *fpoison;             // [P] set by the scatter, read and cleared by the merge
     int fwd_spans;            // 1: the merge stores spans (fused spectral stage, rows not cleared)
-    float *region_l1;         // [num_regions] L1 norm of the region's amplitudes
+    float *region_l1;         // [num_regions] bound on the region's window bin sums (per call)
+    const int *region_cmax;   // [num_regions] pixels per adjacent bin pair, max over pulses
+                              // (once per plan), or NULL: the bound is the region's L1 norm
     int bp_splits;            // backprojection: pulse splits per tile (workgroups per tile)
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
     const double *bp_cu;      // [num_tiles][Pv] series `cu` moved into [shift_lo, shift_lo + nw)
@@ -105,6 +107,10 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
 hipError_t launch_bp_gather(const KernelParams &k, hipStream_t s);
 hipError_t launch_bp_reduce(const KernelParams &k, hipStream_t s);  // pulse-split images
 hipError_t launch_region_l1(const KernelParams &k, hipStream_t s);  // before launch_fp_bin
+// Once per plan, after the pixel records and window sizes: cmax[r] (zeroed by
+// the caller) = largest number of pixels of region r on two adjacent bins of
+// one pulse; 0x7fffffff where a pixel falls outside its window.
+hipError_t launch_region_count(const KernelParams &k, int *cmax, hipStream_t s);
 // Per valid pulse, the bins {first, count} of its adjoint row that the
 // backprojection can read (span[p] for p = rows[j]; other entries untouched).
 hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, const double4 *tiles,

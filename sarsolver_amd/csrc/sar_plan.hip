@@ -141,6 +141,7 @@ struct sarhip_plan_s {
     int *d_fpoison = nullptr;  // [P] rows the scatter could not window (cleared by the merge)
     int2 *d_aspan = nullptr;  // [P] adjoint row spans the backprojection reads (fused path)
     float *d_region_l1 = nullptr;
+    int *d_region_cmax = nullptr;  // [num_regions] see launch_region_count (or NULL)
     double2 *d_bp_partial = nullptr;
     int bp_splits = 1;
 
@@ -185,6 +186,7 @@ struct sarhip_plan_s {
         dev_free(d_fpoison);
         dev_free(d_aspan);
         dev_free(d_region_l1);
+        dev_free(d_region_cmax);
         dev_free(d_bp_partial);
         if (fft_work) (void)hipFree(fft_work);
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
@@ -238,6 +240,7 @@ struct sarhip_plan_s {
         k.fpoison = d_fpoison;
         k.fwd_spans = 0;
         k.region_l1 = d_region_l1;
+        k.region_cmax = d_region_cmax;
         k.bp_splits = bp_splits;
         k.bp_partial = d_bp_partial;
         k.interp = interp;
@@ -890,6 +893,16 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     HIP_TRY(hipMalloc((void **)&pl->d_fpoison, sizeof(int) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMemset(pl->d_fpoison, 0, sizeof(int) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMalloc((void **)&pl->d_region_l1, sizeof(float) * std::max(1, pl->num_regions)));
+    // Pixels per adjacent bin pair of each region (the fixed-point bound of the
+    // forward windows, see region_l1_kernel), for the modes whose ranges agree
+    // with the exact FP64 range far inside the count's 1e-6-bin margin.
+    if (pl->win > 0 && pl->Pv > 0 &&
+        (pl->precision == SARHIP_PRECISION_FP64 || pl->precision == SARHIP_PRECISION_SERIES) &&
+        !env_int("SARHIP_FP_L1_BOUND", 0)) {
+        HIP_TRY(hipMalloc((void **)&pl->d_region_cmax, sizeof(int) * pl->num_regions));
+        HIP_TRY(hipMemset(pl->d_region_cmax, 0, sizeof(int) * pl->num_regions));
+        HIP_TRY(launch_region_count(pl->params(), pl->d_region_cmax, nullptr));
+    }
     // The planar 64-pulse linear scatter streams 32-byte records that the
     // region-norm kernel writes on every forward call.
     if (pl->planar && interp == SARHIP_INTERP_LINEAR && pl->fwd_pulses == 64 &&

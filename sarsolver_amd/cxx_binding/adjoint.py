@@ -7,7 +7,7 @@ import numpy as np
 from numpy.typing import NDArray
 
 from ..base import BaseSarScene, BaseSimpleSarAperture, block_phase_history_array
-from ._plans import device, plan_for
+from ._plans import device, plan_for, plans_for_blocks
 
 
 def _ph_to_device(y: NDArray, dev):
@@ -35,15 +35,15 @@ def multi_adjoint_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarApert
     sequentially and their backprojections are summed; ``num_threads`` has no
     effect on the device path."""
     dev = device()
-    blocks = measurements.to_blocks(num_calculation_blocks)
+    plans = plans_for_blocks(scene, measurements, num_calculation_blocks, upsample_ratio, interp,
+                             precision, dev)
     y_blocks = block_phase_history_array(np.asarray(adjoint_amplitudes), num_calculation_blocks)
-    it = list(zip(blocks, y_blocks))
+    it = list(zip(plans, y_blocks))
     if verbose:
         from tqdm import tqdm
         it = tqdm(it)
     total = None
-    for blk, yb in it:
-        plan = plan_for(scene, blk, upsample_ratio, interp, precision, dev)
+    for plan, yb in it:
         part = plan.adjoint(_ph_to_device(yb, dev), order="centred").cpu().numpy()
         part = part.astype(np.complex128)
         total = part if total is None else total + part

@@ -7,7 +7,7 @@ import numpy as np
 from numpy.typing import NDArray
 
 from ..base import BaseSarScene, BaseSimpleSarAperture
-from ._plans import device, plan_for
+from ._plans import device, plan_for, plans_for_blocks
 
 
 def _to_device_c64(x: NDArray, dev):
@@ -38,14 +38,14 @@ def multi_forward_evaluate(scene: BaseSarScene, measurements: BaseSimpleSarApert
     not split (``num_threads`` is accepted and has no effect: one device launch
     covers every pixel)."""
     dev = device()
-    blocks = measurements.to_blocks(num_calculation_blocks)
+    plans = plans_for_blocks(scene, measurements, num_calculation_blocks, upsample_ratio, interp,
+                             precision, dev)
     x = _to_device_c64(scattering_amplitudes, dev)
-    it = blocks
+    it = plans
     if verbose:
         from tqdm import tqdm
-        it = tqdm(blocks)
+        it = tqdm(plans)
     chunks = []
-    for blk in it:
-        plan = plan_for(scene, blk, upsample_ratio, interp, precision, dev)
+    for plan in it:
         chunks.append(plan.forward(x, order="centred").cpu().numpy().astype(np.complex128))
     return np.concatenate(chunks, axis=0)

@@ -78,6 +78,24 @@ def build(w: Workload, start_deg: float = 0.0):
     return ap, grid_scene(w.n, w.spacing)
 
 
+def workload_plan(w: Workload, ap: NumpySimpleSarAperture, pos, device, interp: str = "linear",
+                  precision: str = "series"):
+    """The device plan of a workload's aperture and pixel positions (as the
+    benchmark builds it: linear interpolation and series ranges by default)."""
+    from ._native import SarPlan
+    return SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
+                   np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting,
+                   ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, w.upsample,
+                   ap.sign_multiplier, interp=interp, precision=precision, device=device)
+
+
+def dense_image(w: Workload, m: int) -> np.ndarray:
+    """The dense complex Gaussian image the benchmark's forward is timed on
+    (every pixel non-zero), seeded from the workload's seed."""
+    g = np.random.default_rng(w.seed + 7)
+    return (g.standard_normal(m) + 1j * g.standard_normal(m)) / np.sqrt(2)
+
+
 def complex_normal(rng: np.random.Generator, shape) -> np.ndarray:
     return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2.0)
 

@@ -113,13 +113,16 @@ def test_gradient_matches_oracle_adjoint(c4, oracle_lib):
                              w.samples, upsample_ratio=w.upsample, interp="linear",
                              num_threads=15)
     eg, ef = rel_l2(g, go), rel_l2(Ax, Axo)
-    # The forward of a smoothed field is a sum that cancels heavily (the field is
-    # nearly invisible to the sensor), so its plain relative error reflects the
-    # ~1e-7 per-term FP32 error amplified by the cancellation; the normwise error
-    # ||dY|| / (||A|| ||x||), with ||A||^2 = P N for unit weights, is the bound.
+    # For this scene's look direction the smoothed field is nearly invisible to
+    # the sensor: its forward is a sum that cancels heavily (the band holds
+    # about 1/60 of the profile's norm), which amplifies per-term errors about
+    # 60-fold.  The plain relative error is held to 1e-5 all the same, and the
+    # normwise error ||dY|| / (||A|| ||x||), ||A||^2 = P N for unit weights,
+    # to 1e-6.
     normwise = np.linalg.norm(Ax - Axo) / (np.sqrt(w.pulses * w.samples)
                                            * np.linalg.norm(X[b].cpu().numpy()))
     print(f"c4 scene {b}: gradient vs oracle {eg:.2e}; forward vs oracle {ef:.2e} "
           f"(normwise {normwise:.2e})")
     assert eg <= 1e-5
-    assert normwise <= 1e-6 and ef <= 1e-4
+    assert ef <= 1e-5
+    assert normwise <= 1e-6

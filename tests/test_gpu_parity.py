@@ -28,16 +28,19 @@ def to_dev(a, dev):
 
 def test_little_backprojection_matches_reference_fixture(dev, little, little_arrays):
     """The reference's only network-free operator pin (reference tests/test_cil.py:12-15):
-    A^H(little_noisy_meas) == little_backproj, here at relative L2 <= 1e-5 and the
-    reference's own elementwise rel 1e-3 on every pixel above 1e-4 of the peak."""
+    A^H(little_noisy_meas) == little_backproj, at relative L2 <= 1e-5 and by the
+    reference's own elementwise criterion, |x - ref| <= 1e-3 |ref| on every
+    pixel (pytest.approx(rel=1e-3)), the smallest pixels (down to 4e-6 of the
+    image peak; 58 pixels lie below 1e-4 of it) included."""
     from sarsolver_amd.cxx_binding import multi_adjoint_evaluate
     ap, sc = little
     print("case: little backprojection U=128", flush=True)
     x = multi_adjoint_evaluate(sc, ap, little_arrays["noisy_meas"]).reshape(sc.grid_shape)
     ref = little_arrays["backproj"]
     assert rel_l2(x, ref) <= 1e-5
-    big = np.abs(ref) > 1e-4 * np.abs(ref).max()
-    assert np.all(np.abs(x - ref)[big] <= 1e-3 * np.abs(ref)[big])
+    ratio = np.abs(x - ref) / np.abs(ref)
+    print(f"little backprojection: worst elementwise |x - ref| / |ref| = {ratio.max():.2e}")
+    assert x.ravel().tolist() == pytest.approx(ref.ravel().tolist(), rel=1e-3)
 
 
 def test_little_pdhg_matches_reference_fixture(dev, little, little_arrays):
@@ -155,3 +158,36 @@ def test_bistatic_weights_odd_n(dev, oracle_lib, interp):
     X = pl.adjoint(to_dev(yv, dev)).cpu().numpy()
     Xo = oracle_lib.adjoint(tx, rx, srp, pos, yv, W, w, fc, fs, upsample_ratio=U, interp=interp)
     assert rel_l2(X, Xo) <= 1e-5
+
+
+@pytest.mark.parametrize("blocks", [2, 3, 5])
+def test_multi_evaluate_pulse_blocks(dev, oracle_lib, blocks):
+    """num_calculation_blocks splits the aperture into pulse blocks
+    (reference forward.py:77-96 concatenates their phase histories,
+    adjoint.py:75-94 sums their backprojections); the result equals the
+    unblocked one up to FP32 rounding and the oracle at 1e-5.  The blocks'
+    plans are cached together: a second call builds no plan."""
+    from sarsolver_amd.cxx_binding import _plans, multi_adjoint_evaluate, multi_forward_evaluate
+    w, ap, sc, x, y = c1_inputs(seed_offset=11)
+    print(f"case: pulse blocks {blocks}", flush=True)
+    kw = dict(upsample_ratio=w.upsample, interp="linear", precision="series")
+    Y1 = multi_forward_evaluate(sc, ap, x, **kw)
+    X1 = multi_adjoint_evaluate(sc, ap, y, **kw)
+    Yb = multi_forward_evaluate(sc, ap, x, num_calculation_blocks=blocks, **kw)
+    Xb = multi_adjoint_evaluate(sc, ap, y, num_calculation_blocks=blocks, **kw)
+    built = _plans.stats["built"]
+    Yb2 = multi_forward_evaluate(sc, ap, x, num_calculation_blocks=blocks, **kw)
+    Xb2 = multi_adjoint_evaluate(sc, ap, y, num_calculation_blocks=blocks, **kw)
+    assert _plans.stats["built"] == built
+    assert np.array_equal(Yb, Yb2) and np.array_equal(Xb, Xb2)
+    # the oracle in FFT order
+    Yo = oracle_forward(oracle_lib, ap, sc.positions, x, w.upsample, "linear")
+    Xo = oracle_adjoint(oracle_lib, ap, sc.positions, np.fft.ifftshift(y, axes=1), w.upsample,
+                        "linear")
+    ef, ea = rel_l2(Yb, Y1), rel_l2(Xb, X1)
+    eo = rel_l2(np.fft.ifftshift(Yb, axes=1), Yo), rel_l2(Xb, Xo)
+    print(f"blocks {blocks}: vs unblocked forward {ef:.2e} adjoint {ea:.2e}; "
+          f"vs oracle {eo[0]:.2e} {eo[1]:.2e}")
+    assert Yb.shape == Y1.shape and Xb.shape == X1.shape
+    assert ef <= 1e-6 and ea <= 1e-6
+    assert max(eo) <= 1e-5

@@ -199,3 +199,50 @@ def test_nonfinite_amplitudes_poison_the_forward(dev, oracle_lib, fused):
                             sc.positions, a, W, w, ap.centre_frequency, ap.sample_bandwidth, 64,
                             upsample_ratio=8.0, interp="linear")
     assert rel_l2(Y, Yo) <= 1e-5
+
+
+@pytest.mark.parametrize("interp", ["nearest_ref", "linear"])
+@pytest.mark.parametrize("exp2", [-110, -60, 60, 100])
+def test_forward_is_linear_at_extreme_scales(dev, interp, exp2):
+    """A(c x) = c A(x) for c = 2^k far from 1.  The fixed-point unit of the
+    scatter's windows and of the merge follows the input's magnitude (scale
+    m 2^e with m in (1, 2]), so a power-of-two c only moves e: the integer sums
+    are identical and the outputs differ by c exactly.  (A scale formed as
+    2^31 / bound overflows FP32 for bounds below ~6e-30, which is where
+    c = 2^-110 puts the regions of this image.)"""
+    from sarsolver_amd import synthetic
+    w = synthetic.WORKLOADS["c1_small"]
+    ap, sc = synthetic.build(w)
+    x = synthetic.dense_image(w, sc.num_scatterers).astype(np.complex64)
+    pl = plan(ap, sc.positions, w.upsample, interp, "series", dev=dev)
+    c = np.float32(2.0 ** exp2)
+    print(f"case: linearity at 2^{exp2} {interp}", flush=True)
+    Y = pl.forward(to_dev(x, dev)).cpu().numpy()
+    Yc = pl.forward(to_dev(x * c, dev)).cpu().numpy()
+    assert np.all(np.isfinite(Yc))
+    e = rel_l2(Yc.astype(np.complex128) / float(c), Y)
+    print(f"2^{exp2} {interp}: rel L2 of A(cx)/c - A(x) = {e:.2e}")
+    assert e <= 1e-6
+
+
+@pytest.mark.parametrize("interp", ["nearest_ref", "linear"])
+def test_forward_after_adjoint_odd_profile_length(dev, oracle_lib, interp):
+    """P N_w odd (P = 7, N = 33, U = 3: 693 profile bins, rocFFT stages with
+    rows cleared before the scatter): the forward that follows an adjoint on
+    the same plan must not see the adjoint's values in the last bin."""
+    from sarsolver_amd import synthetic
+    P, N, U = 7, 33, 3.0
+    ap = synthetic.circular_aperture(P, N, 2.0)
+    sc = synthetic.grid_scene(16, 0.25)
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal(sc.num_scatterers) + 1j * rng.standard_normal(sc.num_scatterers)
+    y = rng.standard_normal((P, N)) + 1j * rng.standard_normal((P, N))
+    pl = plan(ap, sc.positions, U, interp, "fp64", dev=dev)
+    assert pl.layout["fused_fft_ratio"] == 0 and (P * pl.working_num_fast_times) % 2 == 1
+    print(f"case: adjoint then forward, odd P N_w, {interp}", flush=True)
+    X = pl.adjoint(to_dev(y * 1e3, dev)).cpu().numpy()
+    Y = pl.forward(to_dev(a, dev)).cpu().numpy()
+    Yo = oracle_forward(oracle_lib, ap, sc.positions, a, U, interp)
+    Xo = oracle_adjoint(oracle_lib, ap, sc.positions, y * 1e3, U, interp)
+    print(f"odd P N_w {interp}: forward {rel_l2(Y, Yo):.2e} adjoint {rel_l2(X, Xo):.2e}")
+    assert rel_l2(Y, Yo) <= 1e-5 and rel_l2(X, Xo) <= 1e-5

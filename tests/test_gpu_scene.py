@@ -174,7 +174,7 @@ def test_binding_uses_grid_plans(dev, oracle_lib):
     x = g.standard_normal(1600) + 1j * g.standard_normal(1600)
     print("case: bindings with a grid scene", flush=True)
     y1 = multi_forward_evaluate(sc, ap, x, upsample_ratio=8.0)
-    assert any(p.position_source == "grid" for p in _plans._cache.values())
+    assert any(p.position_source == "grid" for ps in _plans._cache.values() for p in ps)
     y2 = multi_forward_evaluate(pts, ap, x, upsample_ratio=8.0)
     assert np.array_equal(y1, y2)
     a1 = multi_adjoint_evaluate(sc, ap, y1, upsample_ratio=8.0)
