@@ -276,6 +276,22 @@ int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, in
                        const void *const *wfrags, const float *const *biases, const int *out_c,
                        void *work, size_t work_bytes, void *out, void *stream);
 
+/* sarhip_precond_net without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight solver or
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments and result as sarhip_precond_net; the workspace
+ * is a static device array of 34 MiB + 4 KiB in the library, enough for
+ * B x H x W up to one 512 x 512 image (sizes whose sarhip_precond_net_workspace
+ * exceeds it are rejected with SARHIP_EINVAL-style -1 and a message).  The
+ * array's address is looked up on the first call on each device (make that
+ * call before capturing a graph); the call itself allocates nothing.  Calls on
+ * one device share the array: calls that overlap on one device (concurrent
+ * streams or threads) race; no state carries over between calls.  The current
+ * device must be the one that holds z, out and `stream`. */
+int sarhip_precond_net_noalloc(const void *z, const void *demod, int B, int H, int W, int layers,
+                               const void *const *wfrags, const float *const *biases,
+                               const int *out_c, void *out, void *stream);
+
 /* Message describing the most recent convolution failure on the calling thread. */
 const char *sarhip_conv_last_error(void);
 

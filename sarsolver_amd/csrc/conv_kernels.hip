@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -297,6 +298,44 @@ NetWork net_workspace(int B, int H, int W) {
 
 thread_local std::string g_conv_err;
 
+// Static workspace of sarhip_precond_net_noalloc: net_workspace(1, 512, 512)
+// = 35 652 352 bytes (two 32-channel bf16 activation buffers, the FP32 output
+// pair, partial sums and scale), rounded up to 34 MiB + 4 KiB.  Each device
+// that loads this library holds one copy for the life of the process; calls
+// on one device share it.
+constexpr size_t kNetScratchBytes = ((size_t)34 << 20) + ((size_t)4 << 10);
+__device__ __align__(256) unsigned char scratch[kNetScratchBytes];
+
+// Device address of `scratch` on the current device, looked up once per
+// device with hipGetSymbolAddress (host code must use this address; the
+// array's name in host code is only a stand-in for the symbol).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+int net_scratch_ptr(char **out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        g_conv_err = std::string("sarhip_precond_net_noalloc: no usable current device: ") +
+                     (e != hipSuccess ? hipGetErrorString(e) : "index out of range");
+        return -2;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_conv_err = std::string("sarhip_precond_net_noalloc: hipGetSymbolAddress: ") +
+                         hipGetErrorString(e);
+            return -2;
+        }
+        g_scratch_ptr[dev] = (char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -377,27 +416,32 @@ size_t sarhip_precond_net_workspace(int B, int H, int W) {
     return net_workspace(B, H, W).total;
 }
 
-int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, int layers,
-                       const void *const *wfrags, const float *const *biases, const int *out_c,
-                       void *work, size_t work_bytes, void *out, void *stream) {
-    if (!z || !out || !wfrags || !biases || !out_c || !work || B <= 0 || H <= 0 || W <= 0 ||
+// Checks shared by the precond_net entry points; 0 or -1 with the message set.
+static int precond_net_check(const void *z, int B, int H, int W, int layers,
+                             const void *const *wfrags, const float *const *biases,
+                             const int *out_c, const void *out, const char *name) {
+    if (!z || !out || !wfrags || !biases || !out_c || B <= 0 || H <= 0 || W <= 0 ||
         layers < 1 || (size_t)B * H * W > ((size_t)1 << 31)) {
-        g_conv_err = "sarhip_precond_net: bad argument";
+        g_conv_err = std::string(name) + ": bad argument";
         return -1;
     }
     for (int i = 0; i < layers; ++i) {
         if (!wfrags[i] || !biases[i] || out_c[i] != (i == layers - 1 ? 2 : kC)) {
-            g_conv_err = "sarhip_precond_net: layer widths must be 32, ..., 32, 2";
+            g_conv_err = std::string(name) + ": layer widths must be 32, ..., 32, 2";
             return -1;
         }
     }
+    return 0;
+}
+
+// The network on device workspace `wk` laid out as net_workspace(B, H, W):
+// every byte the kernels read there is written earlier in the same call, so
+// the workspace needs no clearing between calls.
+static int precond_net_run(const void *z, const void *demod, int B, int H, int W, int layers,
+                           const void *const *wfrags, const float *const *biases,
+                           const int *out_c, char *wk, void *out, void *stream) {
     const NetWork nw = net_workspace(B, H, W);
-    if (work_bytes < nw.total) {
-        g_conv_err = "sarhip_precond_net: workspace too small";
-        return -1;
-    }
     hipStream_t st = (hipStream_t)stream;
-    char *wk = (char *)work;
     const size_t n = (size_t)H * W;
     const int nchunk = (int)((n + kRmsChunk - 1) / kRmsChunk);
     uint16_t *act[2] = {(uint16_t *)(wk + nw.act), (uint16_t *)(wk + nw.act) + (size_t)B * n * kC};
@@ -428,6 +472,39 @@ int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, in
         return -2;
     }
     return 0;
+}
+
+int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, int layers,
+                       const void *const *wfrags, const float *const *biases, const int *out_c,
+                       void *work, size_t work_bytes, void *out, void *stream) {
+    int rc = precond_net_check(z, B, H, W, layers, wfrags, biases, out_c, out,
+                               "sarhip_precond_net");
+    if (rc) return rc;
+    if (!work || work_bytes < net_workspace(B, H, W).total) {
+        g_conv_err = "sarhip_precond_net: workspace missing or too small";
+        return -1;
+    }
+    return precond_net_run(z, demod, B, H, W, layers, wfrags, biases, out_c, (char *)work, out,
+                           stream);
+}
+
+int sarhip_precond_net_noalloc(const void *z, const void *demod, int B, int H, int W, int layers,
+                               const void *const *wfrags, const float *const *biases,
+                               const int *out_c, void *out, void *stream) {
+    int rc = precond_net_check(z, B, H, W, layers, wfrags, biases, out_c, out,
+                               "sarhip_precond_net_noalloc");
+    if (rc) return rc;
+    const size_t need = net_workspace(B, H, W).total;
+    if (need > kNetScratchBytes) {
+        g_conv_err = "sarhip_precond_net_noalloc: B x H x W needs " + std::to_string(need) +
+                     " bytes of workspace, above the static scratch of " +
+                     std::to_string(kNetScratchBytes) +
+                     " bytes (one 512 x 512 image); use sarhip_precond_net with a workspace";
+        return -1;
+    }
+    char *wk = nullptr;
+    if ((rc = net_scratch_ptr(&wk))) return rc;
+    return precond_net_run(z, demod, B, H, W, layers, wfrags, biases, out_c, wk, out, stream);
 }
 
 }  // extern "C"

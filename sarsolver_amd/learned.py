@@ -66,6 +66,9 @@ def _lib():
         L.sarhip_precond_net.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, vp, vp, vp, vp, ctypes.c_size_t, vp, vp]
         L.sarhip_precond_net.restype = ctypes.c_int
+        L.sarhip_precond_net_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_int, vp, vp, vp, vp, vp]
+        L.sarhip_precond_net_noalloc.restype = ctypes.c_int
         L._conv_bound = True
     return L
 
