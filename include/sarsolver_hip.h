@@ -161,6 +161,16 @@ int sarhip_decode_signal(const void *d_raw, int format, size_t num_vectors, size
 
 int sarhip_plan_destroy(sarhip_plan_t plan);
 
+/* Concurrency.  A plan owns mutable device scratch (profile rows, forward
+ * window partials and bounds, row-span and poison flags, pulse-split partial
+ * images) that every sarhip_forward / sarhip_adjoint call on it overwrites.
+ * Calls on ONE plan must therefore be ordered: issue them on one stream, or
+ * order the streams (events / synchronisation) and the host threads yourself;
+ * two calls of one plan that overlap on different streams race and corrupt
+ * each other's results.  Different plans share nothing and may run
+ * concurrently on different streams from different host threads.  Error
+ * messages are per host thread (sarhip_last_error). */
+
 /* Forward projection on device buffers: amps [M] complex64 ->
  * phase_history [P,N] complex64 in `order`.  Stream-ordered on `stream`
  * (a hipStream_t, NULL = default stream).  Device counterpart of the

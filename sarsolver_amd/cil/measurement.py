@@ -1,30 +1,4 @@
-from typing import List
+"""Import path of the reference's cil/measurement.py: the measurement geometry."""
+from .geometry import SimpleCilSarGeometry
 
-import numpy as np
-
-from .base import GenericCilGeometry
-from ..base import BaseSimpleSarAperture
-
-
-class SimpleCilSarGeometry(GenericCilGeometry):
-    """CIL geometry of an aperture; arrays are [P, N] complex128 (cil/measurement.py:9-71)."""
-
-    def __init__(self, aperture: BaseSimpleSarAperture):
-        self._aperture = aperture
-        self._dtype = np.complex128
-
-    @property
-    def aperture(self) -> BaseSimpleSarAperture:
-        return self._aperture
-
-    @property
-    def shape(self) -> List[int]:
-        return [self.aperture.num_slow_times, self.aperture.num_freqs]
-
-    @property
-    def dtype(self) -> type:
-        return self._dtype
-
-    @dtype.setter
-    def dtype(self, val: type):
-        self._dtype = val
+__all__ = ["SimpleCilSarGeometry"]
