@@ -25,8 +25,9 @@ Also reported:
   pcg           time to relative residual 1e-3 for (A^H A + 1e-3 I) x = A^H y
                 (A normalised by its 2-norm) on held-out 20 dB scenes of the
                 512^2 workload (1024 pulses x 2048 samples, 3 deg): plain CGLS,
-                PCG with the demodulated circulant base alone, and PCG with the
-                learned preconditioner (base + bf16 CNN, committed weights);
+                PCG with the analytic demodulated circulant base, PCG with the
+                learned spectral filter alone, and PCG with the learned
+                preconditioner (learned filter + bf16 CNN, committed weights);
                 medians over the scenes and their time ratios to CGLS.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2_large]
 """
@@ -219,8 +220,9 @@ def pcg_comparison(dev, scenes: int):
     w, op = workload_operator("c5_scene", dev)
     n = w.n
     pre = learned.load_preconditioner(path, op, n)
-    res = learned.evaluate(op, {"circulant": pre.base, "learned": lambda r: pre(r, (n, n))}, n,
-                           scenes=scenes)
+    analytic = learned.CirculantPreconditioner(op, n, pre.base.tau, pad=pre.base.pad)
+    res = learned.evaluate(op, {"circulant": analytic, "learned_filter": pre.base,
+                                "learned": lambda r: pre(r, (n, n))}, n, scenes=scenes)
     s = res["solvers"]
     return {
         "metric": "time to 1e-3 residual, learned-PCG vs CGLS",

@@ -286,6 +286,20 @@ int sarhip_precond_net(const void *z, const void *demod, int B, int H, int W, in
                        const void *const *wfrags, const float *const *biases, const int *out_c,
                        void *work, size_t work_bytes, void *out, void *stream);
 
+/* Spectral filtering of the preconditioner's circulant base in one call (no
+ * counterpart in the reference): for each of B images in [n][n] complex64,
+ *   out = demod_out . crop_n( IFFT2_m( filter / m^2 . FFT2_m( pad_m(
+ *         conj(demod_in) . in ) ) ) ),
+ * i.e. the image zero-padded to m x m, filtered by the real filter [m][m]
+ * (float32, indexed [ky][kx] in FFT order), and cropped back; demod_in /
+ * demod_out are [n][n] complex64 phasors or NULL (none).  m a power of two in
+ * [64, 1024], n <= m.  `work`: device workspace of at least
+ * sarhip_spectral_filter_workspace(B, n, m) bytes.  out may alias in. */
+size_t sarhip_spectral_filter_workspace(int B, int n, int m);
+int sarhip_spectral_filter(const void *in, const void *demod_in, int B, int n, int m,
+                           const float *filter, void *work, size_t work_bytes, void *out,
+                           const void *demod_out, void *stream);
+
 /* sarhip_precond_net without a workspace argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight solver or
  * serving loops, harnesses that forbid hipMalloc).  No counterpart in the

@@ -54,7 +54,7 @@ EXPORTED_SYMBOLS = (
     "sarhip_grid_positions", "sarhip_baseband_values", "sarhip_decode_signal",
     "sarhip_conv3x3_bf16", "sarhip_conv3x3_wgrad_workspace", "sarhip_conv3x3_wgrad",
     "sarhip_conv_last_error", "sarhip_precond_net_workspace", "sarhip_precond_net",
-    "sarhip_precond_net_noalloc",
+    "sarhip_precond_net_noalloc", "sarhip_spectral_filter_workspace", "sarhip_spectral_filter",
 )
 
 

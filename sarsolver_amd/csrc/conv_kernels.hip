@@ -26,6 +26,8 @@
 #include <mutex>
 #include <string>
 
+#include "sar_launch.h"
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
@@ -469,6 +471,34 @@ static int precond_net_run(const void *z, const void *demod, int B, int H, int W
     e = hipGetLastError();
     if (e != hipSuccess) {
         g_conv_err = hipGetErrorString(e);
+        return -2;
+    }
+    return 0;
+}
+
+size_t sarhip_spectral_filter_workspace(int B, int n, int m) {
+    if (B <= 0 || n <= 0 || m <= 0) return 0;
+    return sizeof(float2) * (size_t)B * n * m;
+}
+
+int sarhip_spectral_filter(const void *in, const void *demod_in, int B, int n, int m,
+                           const float *filter, void *work, size_t work_bytes, void *out,
+                           const void *demod_out, void *stream) {
+    if (!in || !filter || !work || !out || B <= 0 || !sarhip::spectral_filter_supported(n, m) ||
+        (size_t)B * n * m > ((size_t)1 << 31)) {
+        g_conv_err = "sarhip_spectral_filter: bad argument (need n <= m, m a power of two in "
+                     "[64, 1024])";
+        return -1;
+    }
+    if (work_bytes < sarhip_spectral_filter_workspace(B, n, m)) {
+        g_conv_err = "sarhip_spectral_filter: workspace too small";
+        return -1;
+    }
+    const hipError_t e = sarhip::launch_spectral_filter(
+        (const float2 *)in, (const float2 *)demod_in, B, n, m, filter, (float2 *)work,
+        (float2 *)out, (const float2 *)demod_out, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        g_conv_err = std::string("sarhip_spectral_filter: ") + hipGetErrorString(e);
         return -2;
     }
     return 0;

@@ -430,6 +430,171 @@ size_t lds_bytes(int U) {
     return sizeof(float2) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + (size_t)U * N / 256);
 }
 
+// ---- 2D spectral filter of the learned preconditioner ----------------------
+// out = Dout . crop_n( IFFT2_M( f / M^2 . FFT2_M( pad_M( conj(Din) . in ) ) ) )
+// for B images of n x n complex64 and an M x M real filter (M >= n, a power of
+// two): the circulant preconditioner's zero-padded filtering in three passes
+// -- rows (demodulation, zero padding, forward FFT), columns (forward FFT,
+// filter, inverse FFT, crop), rows (inverse FFT, crop, phasor) -- with one
+// 64-lane wave per length-M transform (the radix-16 Stockham passes above, in
+// LDS) and four transforms per 256-thread workgroup.  Rows of the padding are
+// never transformed, and the intermediate [B][n][M] array holds only the n
+// rows that can be non-zero.
+
+// e^{-2 pi i t / M} = hi[t >> 6] lo[t & 63], each entry rounded once from FP64.
+template <int M>
+struct WaveTw {
+    const float2 *lo, *hi;
+    __device__ __forceinline__ float2 operator()(const int t) const {
+        return cmulf2(hi[t >> 6], lo[t & 63]);
+    }
+};
+template <int M>
+__device__ __forceinline__ WaveTw<M> wave_twiddles(float2 *dst) {
+    for (int i = threadIdx.x; i < 64 + M / 64; i += blockDim.x) {
+        const int t = i < 64 ? i : 64 * (i - 64);
+        double sn, cs;
+        sincospi(-2.0 * (double)t / (double)M, &sn, &cs);
+        dst[i] = make_float2((float)cs, (float)sn);
+    }
+    return WaveTw<M>{dst, dst + 64};
+}
+
+// One Stockham pass of radix R and span Ns of the wave's transform on sh
+// (lanes 0..63 of the wave; every wave of the workgroup reaches both barriers).
+template <int M, int R, int Ns, bool INV>
+__device__ __forceinline__ void wave_pass(float2 *sh, const WaveTw<M> &tw, const int lane) {
+    constexpr int NB = M / R;
+    constexpr int PER = (NB + 63) / 64;
+    float2 v[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = lane + 64 * b;
+        if (j < NB)
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[b][r] = sh[phys(j + r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+        const int j = lane + 64 * b;
+        if (j < NB) {
+            const int jm = j % Ns;
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 t = tw(jm * r * (M / (Ns * R)));
+                    v[b][r] = cmulf2(v[b][r], INV ? conjf2(t) : t);
+                }
+            }
+            dft<R, INV>(v[b]);
+            const int idx = (j / Ns) * Ns * R + jm;
+#pragma unroll
+            for (int r = 0; r < R; ++r) sh[phys(idx + r * Ns)] = v[b][r];
+        }
+    }
+    __syncthreads();
+}
+template <int M, int Ns, bool INV>
+__device__ __forceinline__ void wave_fft(float2 *sh, const WaveTw<M> &tw, const int lane) {
+    if constexpr (Ns < M) {
+        constexpr int rest = M / Ns;
+        constexpr int R = rest >= 16 ? 16 : rest;
+        wave_pass<M, R, Ns, INV>(sh, tw, lane);
+        wave_fft<M, Ns * R, INV>(sh, tw, lane);
+    }
+}
+template <int M>
+constexpr int kSfSlot = M + M / 16;  // LDS entries of one transform (padded)
+template <int M>
+constexpr size_t kSfLds = sizeof(float2) * (4 * (size_t)kSfSlot<M> + 64 + M / 64);
+
+// rows: T[b][y][kx] = FFT_M(pad_M(conj(Din) in[b][y]))[kx], y < n
+template <int M>
+__global__ __launch_bounds__(256) void sf_rows_fwd_kernel(const float2 *__restrict__ in,
+                                                          const float2 *__restrict__ din, int n,
+                                                          float2 *__restrict__ T) {
+    extern __shared__ float2 lds[];
+    const WaveTw<M> tw = wave_twiddles<M>(lds + 4 * kSfSlot<M>);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y = blockIdx.x * 4 + w, b = blockIdx.y;
+    const bool live = y < n;
+    float2 *sh = lds + w * kSfSlot<M>;
+    const float2 *row = in + ((size_t)b * n + (live ? y : 0)) * n;
+    for (int x = lane; x < M; x += 64) {
+        float2 v = make_float2(0.f, 0.f);
+        if (live && x < n) {
+            v = row[x];
+            if (din) v = cmulf2(v, conjf2(din[(size_t)y * n + x]));
+        }
+        sh[phys(x)] = v;
+    }
+    __syncthreads();
+    wave_fft<M, 1, false>(sh, tw, lane);
+    if (live) {
+        float2 *dst = T + ((size_t)b * n + y) * M;
+        for (int k = lane; k < M; k += 64) dst[k] = sh[phys(k)];
+    }
+}
+
+// columns: T[b][y][kx] (y < n) <- crop_n IFFT_M( f[.][kx] scale FFT_M(pad_M T[b][.][kx]) )
+template <int M>
+__global__ __launch_bounds__(256) void sf_cols_kernel(float2 *__restrict__ T, int n,
+                                                      const float *__restrict__ f, float scale) {
+    extern __shared__ float2 lds[];
+    const WaveTw<M> tw = wave_twiddles<M>(lds + 4 * kSfSlot<M>);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int kx = blockIdx.x * 4 + w, b = blockIdx.y;
+    float2 *sh = lds + w * kSfSlot<M>;
+    float2 *col = T + (size_t)b * n * M + kx;
+    for (int y = lane; y < M; y += 64) sh[phys(y)] = y < n ? col[(size_t)y * M] : make_float2(0.f, 0.f);
+    __syncthreads();
+    wave_fft<M, 1, false>(sh, tw, lane);
+    for (int k = lane; k < M; k += 64) {
+        const float g = f[(size_t)k * M + kx] * scale;
+        const float2 v = sh[phys(k)];
+        sh[phys(k)] = make_float2(v.x * g, v.y * g);
+    }
+    __syncthreads();
+    wave_fft<M, 1, true>(sh, tw, lane);
+    for (int y = lane; y < n; y += 64) col[(size_t)y * M] = sh[phys(y)];
+}
+
+// rows back: out[b][y][x] = Dout[y][x] IFFT_M(T[b][y])[x], x < n
+template <int M>
+__global__ __launch_bounds__(256) void sf_rows_inv_kernel(const float2 *__restrict__ T, int n,
+                                                          const float2 *__restrict__ dout,
+                                                          float2 *__restrict__ out) {
+    extern __shared__ float2 lds[];
+    const WaveTw<M> tw = wave_twiddles<M>(lds + 4 * kSfSlot<M>);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int y = blockIdx.x * 4 + w, b = blockIdx.y;
+    const bool live = y < n;
+    float2 *sh = lds + w * kSfSlot<M>;
+    const float2 *src = T + ((size_t)b * n + (live ? y : 0)) * M;
+    for (int k = lane; k < M; k += 64) sh[phys(k)] = live ? src[k] : make_float2(0.f, 0.f);
+    __syncthreads();
+    wave_fft<M, 1, true>(sh, tw, lane);
+    if (live) {
+        float2 *dst = out + ((size_t)b * n + y) * n;
+        for (int x = lane; x < n; x += 64) {
+            float2 v = sh[phys(x)];
+            if (dout) v = cmulf2(v, dout[(size_t)y * n + x]);
+            dst[x] = v;
+        }
+    }
+}
+
+template <int M>
+hipError_t spectral_filter(const float2 *in, const float2 *din, int B, int n, const float *f,
+                           float2 *T, float2 *out, const float2 *dout, hipStream_t s) {
+    const dim3 rows((unsigned)((n + 3) / 4), (unsigned)B), cols((unsigned)(M / 4), (unsigned)B);
+    sf_rows_fwd_kernel<M><<<rows, 256, kSfLds<M>, s>>>(in, din, n, T);
+    sf_cols_kernel<M><<<cols, 256, kSfLds<M>, s>>>(T, n, f, 1.0f / ((float)M * (float)M));
+    sf_rows_inv_kernel<M><<<rows, 256, kSfLds<M>, s>>>(T, n, dout, out);
+    return hipGetLastError();
+}
+
 template <int N, int Q>
 hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac, int U,
               int ld, int P, int centred, const float2 *tables, float2 *prof, hipStream_t s) {
@@ -514,6 +679,23 @@ hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *
     SARHIP_FFT_SIZES(DOWN_CASE)
 #undef DOWN_CASE
     return hipErrorInvalidValue;
+}
+
+bool spectral_filter_supported(int n, int M) {
+    return n >= 1 && n <= M && M >= 64 && M <= 1024 && (M & (M - 1)) == 0;
+}
+
+hipError_t launch_spectral_filter(const float2 *in, const float2 *din, int B, int n, int M,
+                                  const float *f, float2 *work, float2 *out, const float2 *dout,
+                                  hipStream_t s) {
+    switch (M) {
+        case 64: return spectral_filter<64>(in, din, B, n, f, work, out, dout, s);
+        case 128: return spectral_filter<128>(in, din, B, n, f, work, out, dout, s);
+        case 256: return spectral_filter<256>(in, din, B, n, f, work, out, dout, s);
+        case 512: return spectral_filter<512>(in, din, B, n, f, work, out, dout, s);
+        case 1024: return spectral_filter<1024>(in, din, B, n, f, work, out, dout, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace sarhip

@@ -142,6 +142,13 @@ hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables, float2 *Y,
                            bool sparse, hipStream_t s);
+// spectral filter of the learned preconditioner (fft_kernels.hip): out =
+// dout . crop_n IFFT2_M(f / M^2 . FFT2_M(pad_M(conj(din) . in))), B images n x n,
+// filter f [M][M] float (ky, kx), work >= B n M complex64, din / dout optional
+bool spectral_filter_supported(int n, int M);
+hipError_t launch_spectral_filter(const float2 *in, const float2 *din, int B, int n, int M,
+                                  const float *f, float2 *work, float2 *out, const float2 *dout,
+                                  hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
 hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
 

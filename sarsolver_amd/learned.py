@@ -69,6 +69,11 @@ def _lib():
         L.sarhip_precond_net_noalloc.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                  ctypes.c_int, vp, vp, vp, vp, vp]
         L.sarhip_precond_net_noalloc.restype = ctypes.c_int
+        L.sarhip_spectral_filter_workspace.argtypes = [ctypes.c_int] * 3
+        L.sarhip_spectral_filter_workspace.restype = ctypes.c_size_t
+        L.sarhip_spectral_filter.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                                             vp, ctypes.c_size_t, vp, vp, vp]
+        L.sarhip_spectral_filter.restype = ctypes.c_int
         L._conv_bound = True
     return L
 
@@ -265,7 +270,7 @@ class CnnPreconditioner:
         one library call (scale, bf16 packing, the layers, output and phasor)."""
         flat = r.dim() == 1
         b = self.base
-        zb = r if b is None else b.apply_base(b.to_base(r))
+        zb = r if b is None else b.base_coords(r)
         z = zb.reshape((-1,) + tuple(shape))
         if not torch.is_grad_enabled() and self._fused():
             demod = None if b is None else b.demod
@@ -338,7 +343,16 @@ class CirculantPreconditioner:
     (the near-field look direction moves the band across the scene).
     sigma(k) is the spectrum of D^H K D delta for a delta at the image centre,
     re-centred at the origin.  P is Hermitian positive definite; tau >= lam
-    regularises the band edges and out-of-band frequencies."""
+    regularises the band edges and out-of-band frequencies.
+
+    Learned spectral gain (optional): the filter becomes
+    exp(g(k)) / (sigma(k) + tau) with g a real log-gain held on a (pad/2)^2
+    grid and interpolated bilinearly to the padded grid; exp keeps P
+    Hermitian positive definite.  g is trained by backpropagating through
+    unrolled PCG (`train_filter`); it corrects what a single shift-invariant
+    filter gets wrong -- the band edges move across the scene (the look
+    direction changes) and the image borders truncate the point-spread
+    function -- as well as one filter can, at no cost per application."""
 
     def __init__(self, op, side: int, tau: float, pad: Optional[int] = None,
                  demodulate: bool = True, phase: Optional[np.ndarray] = None):
@@ -369,8 +383,10 @@ class CirculantPreconditioner:
             hp[:n, :n] = h
             hp = torch.roll(hp, shifts=(-(n // 2), -(n // 2)), dims=(0, 1))
             sig = torch.fft.fft2(hp).real.clamp_min(0.0)
-            self.filter = (1.0 / (sig + tau)).to(torch.float32)
+            self.base_filter = (1.0 / (sig + tau)).to(torch.float32)
             self.sigma = sig
+        self.log_gain: Optional[torch.Tensor] = None  # [pad/2, pad/2] or None
+        self._filter_cache = None
 
     @staticmethod
     def _psf(op, d, n):
@@ -382,6 +398,76 @@ class CirculantPreconditioner:
         if d is not None:
             h = h * d.conj()
         return h.reshape(n, n)
+
+    # -- learned spectral gain ------------------------------------------------
+    def learn_gain(self) -> torch.Tensor:
+        """Make the log-gain a trainable tensor (zeros: P unchanged) and return it."""
+        if self.log_gain is None:
+            h = (self.pad + 1) // 2
+            self.log_gain = torch.zeros(h, h, dtype=torch.float32, device=self.base_filter.device)
+        self.log_gain.requires_grad_(True)
+        self._filter_cache = None
+        return self.log_gain
+
+    def set_gain(self, log_gain) -> None:
+        """Install a (pad/2)^2 log-gain (numpy or tensor), frozen."""
+        t = torch.as_tensor(np.asarray(log_gain, dtype=np.float32)).to(self.base_filter.device)
+        h = (self.pad + 1) // 2
+        if tuple(t.shape) != (h, h):
+            raise ValueError(f"log-gain shape {tuple(t.shape)}, expected {(h, h)}")
+        self.log_gain = t
+        self._filter_cache = None
+
+    def _gain(self) -> torch.Tensor:
+        g = self.log_gain[None, None]
+        m = self.pad
+        return torch.nn.functional.interpolate(g, size=(m, m), mode="bilinear",
+                                               align_corners=False)[0, 0]
+
+    @property
+    def filter(self) -> torch.Tensor:
+        """The spectral filter on the padded grid, [pad, pad] float32."""
+        if self.log_gain is None:
+            return self.base_filter
+        if self.log_gain.requires_grad and torch.is_grad_enabled():
+            return self.base_filter * torch.exp(self._gain())
+        key = self.log_gain._version
+        if self._filter_cache is None or self._filter_cache[0] != key:
+            with torch.no_grad():
+                self._filter_cache = (key, (self.base_filter * torch.exp(self._gain())).contiguous())
+        return self._filter_cache[1]
+
+    # -- application ----------------------------------------------------------
+    def _fused(self, r: torch.Tensor, phasor_out: bool) -> Optional[torch.Tensor]:
+        """The whole filtering in one library call (sarhip_spectral_filter:
+        demodulation, zero padding, 2D FFT, filter, inverse FFT, crop and the
+        optional output phasor) when no gradient is needed; None otherwise."""
+        if torch.is_grad_enabled() or not r.is_cuda or r.dtype != torch.complex64:
+            return None
+        n, m = self.side, self.pad
+        if not (m >= 64 and m <= 1024 and (m & (m - 1)) == 0 and n <= m):
+            return None
+        z = r.reshape(-1, n, n).contiguous()
+        B = z.shape[0]
+        L = _lib()
+        key = (B, n, m, z.device)
+        if getattr(self, "_sf_key", None) != key:
+            self._sf_work = torch.empty(int(L.sarhip_spectral_filter_workspace(B, n, m)),
+                                        dtype=torch.uint8, device=z.device)
+            self._sf_key = key
+        f = self.filter.contiguous()
+        out = torch.empty_like(z)
+        d = None if self.demod is None else _ptr(self.demod)
+        rc = L.sarhip_spectral_filter(_ptr(z), d, B, n, m, _ptr(f), _ptr(self._sf_work),
+                                      self._sf_work.numel(), _ptr(out), d if phasor_out else None,
+                                      _stream(z.device))
+        _conv_check(rc, "sarhip_spectral_filter")
+        return out.reshape(r.shape)
+
+    def base_coords(self, r: torch.Tensor) -> torch.Tensor:
+        """P's output before the final phasor: apply_base(to_base(r))."""
+        out = self._fused(r, phasor_out=False)
+        return out if out is not None else self.apply_base(self.to_base(r))
 
     def to_base(self, r: torch.Tensor) -> torch.Tensor:
         return r if self.demod is None else r * self.demod.conj()
@@ -402,7 +488,8 @@ class CirculantPreconditioner:
         return out.reshape(z.shape).to(z.dtype)
 
     def __call__(self, r: torch.Tensor, shape=None) -> torch.Tensor:
-        return self.from_base(self.apply_base(self.to_base(r)))
+        out = self._fused(r, phasor_out=True)
+        return out if out is not None else self.from_base(self.apply_base(self.to_base(r)))
 
 
 @dataclass
@@ -510,6 +597,37 @@ def train_unrolled(op, pre: CnnPreconditioner, side: int, lam: float = 1e-3, ite
     return pre, {"history": history, "seconds": time.perf_counter() - t0}
 
 
+def train_filter(op, base: CirculantPreconditioner, side: int, lam: float = 1e-3,
+                 iters: int = 12, steps: int = 1000, batch: int = 2, lr: float = 0.02,
+                 seed: int = 5, log_every: int = 100, progress=None):
+    """Train the base's spectral log-gain through `iters` unrolled PCG
+    iterations on fresh random problems (loss `unrolled_pcg_loss` with the
+    base alone as M), Adam with cosine decay."""
+    g = base.learn_gain()
+    opt = torch.optim.Adam([g], lr=lr)
+    sched = torch.optim.lr_scheduler.CosineAnnealingLR(opt, steps)
+    gens = make_gens(seed, op.device)
+    history = []
+    t0 = time.perf_counter()
+    for step in range(steps):
+        c = problem_batch(op, side, batch, gens)
+        loss = unrolled_pcg_loss(op, base, c, lam, iters)
+        opt.zero_grad()
+        loss.backward()
+        opt.step()
+        sched.step()
+        if step % log_every == 0 or step == steps - 1:
+            lv = float(loss.detach())
+            history.append((step, lv))
+            if progress is not None:
+                progress(step, lv, time.perf_counter() - t0)
+    g.requires_grad_(False)
+    base._filter_cache = None
+    if op.device.type == "cuda":
+        torch.cuda.synchronize(op.device)
+    return {"history": history, "seconds": time.perf_counter() - t0}
+
+
 def choose_tau(op, side: int, taus, problems, lam: float = 1e-3, tol: float = 1e-3,
                pad: Optional[int] = None, max_iter: int = 300):
     """The tau of `CirculantPreconditioner` with the fewest median PCG
@@ -568,20 +686,27 @@ def evaluate(op, preconds: Dict[str, object], side: int, scenes: int = 64,
 
 
 def save_preconditioner(path, pre: CnnPreconditioner, meta: dict) -> None:
-    """Network weights (FP32 master copies) and the base's settings as .npz."""
+    """Network weights (FP32 master copies), the base's learned spectral
+    log-gain (float16, when trained) and the base's settings as .npz."""
     arrays = {k: v.numpy() for k, v in pre.state_dict().items()}
+    if pre.base is not None and pre.base.log_gain is not None:
+        arrays["log_gain"] = pre.base.log_gain.detach().cpu().numpy().astype(np.float16)
     arrays["meta_keys"] = np.array(list(meta.keys()))
     arrays["meta_vals"] = np.array([float(v) for v in meta.values()])
     np.savez(path, **arrays)
 
 
 def load_preconditioner(path, op, side: int) -> CnnPreconditioner:
-    """Rebuild M = D [z + s net(z / s)] for `op` from `save_preconditioner` output."""
+    """Rebuild M = D [z + s net(z / s)], z = P D^H r, for `op` from
+    `save_preconditioner` output (P with its learned gain when stored)."""
     with np.load(Path(path), allow_pickle=False) as f:
         meta = dict(zip([str(k) for k in f["meta_keys"]], f["meta_vals"].tolist()))
         sd = {k: torch.from_numpy(f[k]) for k in f.files if k[0] in "wb" and k[1:].isdigit()}
+        gain = f["log_gain"].astype(np.float32) if "log_gain" in f.files else None
     pad = int(meta.get("pad", side))
     base = CirculantPreconditioner(op, side, meta["tau"], pad=pad)
+    if gain is not None:
+        base.set_gain(gain)
     pre = CnnPreconditioner(op.device, base=base)
     pre.load_state_dict(sd)
     return pre

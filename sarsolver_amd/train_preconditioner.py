@@ -7,11 +7,16 @@ Steps (all on the GPU, through the project's own SAR kernels):
 1. plan for the workload (512^2 scene, 1024 pulses x 2048 samples, 3 deg,
    U = 8, linear interpolation, series ranges) and Â = A / ||A||_2;
 2. tau of the demodulated circulant base chosen on 8 training problems;
-3. the six-layer bf16 CNN trained through 12 unrolled PCG iterations;
-4. weights written to --out (FP32 .npz, with tau, pad and training settings);
-5. CGLS, PCG with the circulant base alone and PCG with the learned M on
-   --eval-scenes held-out 20 dB scenes, to relative residual 1e-3; the
-   summary (medians, time ratios, worst true residuals) printed as JSON.
+3. the base's spectral log-gain trained through 12 unrolled PCG iterations
+   (--filter-steps problems of --batch scenes);
+4. the six-layer bf16 CNN trained on top of it through 12 unrolled PCG
+   iterations (--steps problems of --batch scenes);
+5. weights written to --out (.npz: FP32 network, float16 log-gain, tau, pad
+   and training settings);
+6. CGLS, PCG with the analytic circulant base, PCG with the learned filter
+   alone and PCG with the learned filter + CNN on --eval-scenes held-out 20 dB
+   scenes, to relative residual 1e-3; the summary (medians, time ratios,
+   worst true residuals) printed as JSON.
 """
 from __future__ import annotations
 
@@ -50,10 +55,12 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--workload", default="c5_scene")
     ap.add_argument("--out", default=None)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=600, help="CNN training steps")
+    ap.add_argument("--filter-steps", type=int, default=1000, help="spectral-gain training steps")
+    ap.add_argument("--filter-lr", type=float, default=0.02)
     ap.add_argument("--iters", type=int, default=12)
     ap.add_argument("--batch", type=int, default=2)
-    ap.add_argument("--lr", type=float, default=1e-4)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--eval-scenes", type=int, default=64)
     ap.add_argument("--summary", default=None, help="also write the summary JSON here")
     ap.add_argument("--init", default=None,
@@ -80,32 +87,47 @@ def main(argv=None) -> int:
         print(f"tau scores (median PCG iterations): {scores} -> tau {tau}", flush=True)
         base = learned.CirculantPreconditioner(op, n, tau, pad=PAD_FACTOR * n)
         pre = learned.CnnPreconditioner(dev, base=base)
+    analytic = learned.CirculantPreconditioner(op, n, tau, pad=PAD_FACTOR * n)
     cval = learned.problem_batch(op, n, 4, learned.make_gens(4242, dev))
+    prog = lambda st, l, el: print(f"  step {st} loss {l:.4f} ({el:.1f} s)", flush=True)
+    with torch.no_grad():
+        la = float(learned.unrolled_pcg_loss(op, analytic, cval, lam, a.iters))
+    finfo = {"seconds": 0.0}
+    if a.filter_steps > 0:
+        print(f"spectral gain: {a.filter_steps} steps", flush=True)
+        finfo = learned.train_filter(op, base, n, lam=lam, iters=a.iters, steps=a.filter_steps,
+                                     batch=a.batch, lr=a.filter_lr, seed=a.seed + 1000,
+                                     log_every=max(1, a.filter_steps // 10), progress=prog)
     with torch.no_grad():
         lb = float(learned.unrolled_pcg_loss(op, base, cval, lam, a.iters))
         l0 = float(learned.unrolled_pcg_loss(op, lambda r: pre(r, (n, n)), cval, lam, a.iters))
-    prog = lambda st, l, el: print(f"  step {st} loss {l:.4f} ({el:.1f} s)", flush=True)
+    print(f"CNN: {a.steps} steps", flush=True)
     pre, info = learned.train_unrolled(op, pre, n, lam=lam, iters=a.iters, steps=a.steps,
                                        batch=a.batch, lr=a.lr, seed=a.seed,
                                        log_every=max(1, a.steps // 20), progress=prog)
     with torch.no_grad():
         l1 = float(learned.unrolled_pcg_loss(op, lambda r: pre(r, (n, n)), cval, lam, a.iters))
-    print(f"held-out unrolled loss (mean log10 residual over {a.iters} its): circulant "
-          f"base alone {lb:.4f}; network before {l0:.4f} -> after {l1:.4f}", flush=True)
+    print(f"held-out unrolled loss (mean log10 residual over {a.iters} its): analytic "
+          f"circulant {la:.4f}; learned filter {lb:.4f}; network before {l0:.4f} -> after "
+          f"{l1:.4f}", flush=True)
     out = Path(a.out) if a.out else default_weights(a.workload)
     out.parent.mkdir(parents=True, exist_ok=True)
     learned.save_preconditioner(out, pre, {
         "tau": tau, "pad": PAD_FACTOR * n, "steps": a.steps, "iters": a.iters, "lr": a.lr,
         "batch": a.batch, "seed": a.seed, "lam": lam, "train_seconds": info["seconds"],
+        "filter_steps": a.filter_steps, "filter_lr": a.filter_lr,
+        "filter_seconds": finfo["seconds"], "loss_analytic": la,
         "loss_base": lb, "loss_start": l0, "loss_learned": l1})
     print(f"weights -> {out}", flush=True)
     if a.no_eval:
         return 0
     t = time.perf_counter()
-    res = learned.evaluate(op, {"circulant": base, "learned": lambda r: pre(r, (n, n))}, n,
+    res = learned.evaluate(op, {"circulant": analytic, "learned_filter": base,
+                                "learned": lambda r: pre(r, (n, n))}, n,
                            scenes=a.eval_scenes, lam=lam)
     summary = {k: v for k, v in res.items() if k != "rows"}
     summary.update({"workload": a.workload, "tau": tau, "train_seconds": info["seconds"],
+                    "filter_seconds": finfo["seconds"], "loss_analytic": la,
                     "loss_base": lb, "loss_start": l0, "loss_learned": l1,
                     "eval_seconds": time.perf_counter() - t})
     print(json.dumps(summary), flush=True)
