@@ -28,6 +28,28 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // no swizzle, no stride).
 constexpr int kBufferWord3 = 0x00020000;
 
+// Structured-buffer load (LLVM's struct buffer intrinsic; HIP has no builtin
+// for it): 16 bytes at base + stride * vindex + voffset, range-checked on the
+// index alone -- vindex >= num_records reads zeros, any smaller index reads all
+// 16 bytes (checked on MI355X with stride 8: index num_records - 1 returns its
+// record and the next one).
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+__device__ i32x4 llvm_struct_buffer_load_b128(i32x4 rsrc, int vindex, int voffset, int soffset,
+                                              int aux) __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+// Resource over one profile row of `ld` complex64 bins as records of 8 bytes,
+// num_records = ld - 1: a load at bin n reads bins n and n + 1 for n <= ld - 2
+// and zeros beyond, so it never reads past the row.  The bin index goes to the
+// load as is (no byte-offset shift per sample).
+__device__ __forceinline__ i32x4 row_pair_rsrc(const float2 *row, const int ld) {
+    const unsigned long long a = (unsigned long long)row;
+    i32x4 r;
+    r.x = (int)(unsigned)a;
+    r.y = (int)(((unsigned)(a >> 32) & 0xffffu) | (8u << 16));  // stride 8 bytes
+    r.z = ld - 1;
+    r.w = kBufferWord3;
+    return r;
+}
+
 // Two consecutive profile samples (8-byte aligned) read as one 16-byte load.
 struct alignas(8) Float2x2 {
     float a, b, c, d;
@@ -439,10 +461,9 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
             }
         }
         const float2 *row = k.prof + (size_t)p * k.ld;
-        // 16-byte loads of bins (n, n + 1) through a buffer resource over the
-        // padded row: offsets beyond it read zeros instead of faulting.
-        const __amdgpu_buffer_rsrc_t rsrc =
-            __builtin_amdgcn_make_buffer_rsrc((void *)row, (short)0, k.ld * 8, kBufferWord3);
+        // 16-byte loads of bins (n, n + 1) through a structured buffer over the
+        // padded row: indices beyond it read zeros instead of faulting.
+        const i32x4 rsrc = row_pair_rsrc(row, k.ld);
         if constexpr (INTERP == kInterpLinear && kShift) {
             // all pixels' bins and phases first, so that their profile loads
             // are in flight together, then interpolation and accumulation
@@ -461,7 +482,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
                 // u > 0: truncation is floor, v_fract_f64 the exact rest
                 const int n0 = (int)u;
                 fr[m] = (float)__builtin_amdgcn_fract(u);
-                h[m] = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
+                h[m] = (u32x4)llvm_struct_buffer_load_b128(rsrc, n0, 0, 0, 0);
                 phasor_fract(th, &sn[m], &cs[m]);
             }
 #pragma unroll
@@ -483,7 +504,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
                 const double fl = floor(u);
                 const float fr = (float)(u - fl);
                 const int n0 = bin_index(k, fl, 1);
-                const u32x4 h = __builtin_amdgcn_raw_buffer_load_b128(rsrc, n0 * 8, 0, 0);
+                const u32x4 h = (u32x4)llvm_struct_buffer_load_b128(rsrc, n0, 0, 0, 0);
                 const float h0x = __uint_as_float(h.x), h0y = __uint_as_float(h.y);
                 hr = fmaf(fr, __uint_as_float(h.z) - h0x, h0x);
                 hi = fmaf(fr, __uint_as_float(h.w) - h0y, h0y);
