@@ -696,15 +696,16 @@ def save_preconditioner(path, pre: CnnPreconditioner, meta: dict) -> None:
     np.savez(path, **arrays)
 
 
-def load_preconditioner(path, op, side: int) -> CnnPreconditioner:
+def load_preconditioner(path, op, side: int, phase: Optional[np.ndarray] = None) -> CnnPreconditioner:
     """Rebuild M = D [z + s net(z / s)], z = P D^H r, for `op` from
-    `save_preconditioner` output (P with its learned gain when stored)."""
+    `save_preconditioner` output (P with its learned gain when stored; the
+    carrier phase of D as `CirculantPreconditioner` takes it)."""
     with np.load(Path(path), allow_pickle=False) as f:
         meta = dict(zip([str(k) for k in f["meta_keys"]], f["meta_vals"].tolist()))
         sd = {k: torch.from_numpy(f[k]) for k in f.files if k[0] in "wb" and k[1:].isdigit()}
         gain = f["log_gain"].astype(np.float32) if "log_gain" in f.files else None
     pad = int(meta.get("pad", side))
-    base = CirculantPreconditioner(op, side, meta["tau"], pad=pad)
+    base = CirculantPreconditioner(op, side, meta["tau"], pad=pad, phase=phase)
     if gain is not None:
         base.set_gain(gain)
     pre = CnnPreconditioner(op.device, base=base)

@@ -55,8 +55,8 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--workload", default="c5_scene")
     ap.add_argument("--out", default=None)
-    ap.add_argument("--steps", type=int, default=600, help="CNN training steps")
-    ap.add_argument("--filter-steps", type=int, default=1000, help="spectral-gain training steps")
+    ap.add_argument("--steps", type=int, default=2500, help="CNN training steps")
+    ap.add_argument("--filter-steps", type=int, default=2500, help="spectral-gain training steps")
     ap.add_argument("--filter-lr", type=float, default=0.02)
     ap.add_argument("--iters", type=int, default=12)
     ap.add_argument("--batch", type=int, default=2)
