@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <type_traits>
 
 #include "sar_device.h"
 #include "sar_launch.h"
@@ -34,50 +35,74 @@ __device__ __forceinline__ float2 addf2(const float2 a, const float2 b) {
 __device__ __forceinline__ float2 subf2(const float2 a, const float2 b) {
     return make_float2(a.x - b.x, a.y - b.y);
 }
+// The transform core below is written once for FP32 (float2) and FP64
+// (double2) complex values through these overloads.
+__device__ __forceinline__ float2 cmul(const float2 a, const float2 b) { return cmulf2(a, b); }
+__device__ __forceinline__ float2 cconj(const float2 a) { return conjf2(a); }
+__device__ __forceinline__ float2 cadd(const float2 a, const float2 b) { return addf2(a, b); }
+__device__ __forceinline__ float2 csub(const float2 a, const float2 b) { return subf2(a, b); }
+__device__ __forceinline__ double2 cmul(const double2 a, const double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ double2 cconj(const double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cadd(const double2 a, const double2 b) {
+    return make_double2(a.x + b.x, a.y + b.y);
+}
+__device__ __forceinline__ double2 csub(const double2 a, const double2 b) {
+    return make_double2(a.x - b.x, a.y - b.y);
+}
+template <typename C>
+__device__ __forceinline__ C cmake(const double x, const double y) {
+    C c;
+    c.x = x;
+    c.y = y;
+    return c;
+}
 // i * a (INV) or -i * a
-template <bool INV>
-__device__ __forceinline__ float2 rot(const float2 a) {
-    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+template <bool INV, typename C>
+__device__ __forceinline__ C rot(const C a) {
+    return INV ? cmake<C>(-a.y, a.x) : cmake<C>(a.y, -a.x);
 }
 
 // LDS index with one pad slot per 16 (spreads the radix-16 writes over banks).
 __device__ __forceinline__ int phys(const int i) { return i + (i >> 4); }
 
 // In-register DFTs, out[q] = sum_r v[r] e^{sign 2 pi i r q / R}, sign = +1 for INV.
-template <bool INV>
-__device__ __forceinline__ void dft2(float2 &a, float2 &b) {
-    const float2 t = a;
-    a = addf2(t, b);
-    b = subf2(t, b);
+template <bool INV, typename C>
+__device__ __forceinline__ void dft2(C &a, C &b) {
+    const C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
 }
-template <bool INV>
-__device__ __forceinline__ void dft4(float2 &a, float2 &b, float2 &c, float2 &d) {
-    const float2 s0 = addf2(a, c), d0 = subf2(a, c);
-    const float2 s1 = addf2(b, d), d1 = rot<INV>(subf2(b, d));
-    a = addf2(s0, s1);
-    c = subf2(s0, s1);
-    b = addf2(d0, d1);
-    d = subf2(d0, d1);
+template <bool INV, typename C>
+__device__ __forceinline__ void dft4(C &a, C &b, C &c, C &d) {
+    const C s0 = cadd(a, c), d0 = csub(a, c);
+    const C s1 = cadd(b, d), d1 = rot<INV>(csub(b, d));
+    a = cadd(s0, s1);
+    c = csub(s0, s1);
+    b = cadd(d0, d1);
+    d = csub(d0, d1);
 }
 
 // e^{sign 2 pi i j / 16}, j = 0..9 (the products b c of the 4 x 4 split).
-__device__ __forceinline__ float2 w16(const int j, const bool inv) {
-    constexpr float C1 = 0.92387953251128674f, S1 = 0.38268343236508978f, H = 0.70710678118654752f;
-    float2 w;
+template <typename C>
+__device__ __forceinline__ C w16(const int j, const bool inv) {
+    constexpr double C1 = 0.92387953251128674, S1 = 0.38268343236508978, H = 0.70710678118654752;
+    C w;
     switch (j) {
-        case 0: w = make_float2(1.f, 0.f); break;
-        case 1: w = make_float2(C1, S1); break;
-        case 2: w = make_float2(H, H); break;
-        case 3: w = make_float2(S1, C1); break;
-        case 4: w = make_float2(0.f, 1.f); break;
-        case 6: w = make_float2(-H, H); break;
-        default: w = make_float2(-C1, -S1); break;  // j = 9
+        case 0: w = cmake<C>(1.0, 0.0); break;
+        case 1: w = cmake<C>(C1, S1); break;
+        case 2: w = cmake<C>(H, H); break;
+        case 3: w = cmake<C>(S1, C1); break;
+        case 4: w = cmake<C>(0.0, 1.0); break;
+        case 6: w = cmake<C>(-H, H); break;
+        default: w = cmake<C>(-C1, -S1); break;  // j = 9
     }
-    return inv ? w : conjf2(w);
+    return inv ? w : cconj(w);
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void dft(float2 (&v)[R]) {
+template <int R, bool INV, typename C>
+__device__ __forceinline__ void dft(C (&v)[R]) {
     if constexpr (R == 2) {
         dft2<INV>(v[0], v[1]);
     } else if constexpr (R == 4) {
@@ -86,14 +111,14 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
         // r = 2a + b, q = c + 4d: DFT4 over a, twiddle e^{sign 2 pi i b c / 8}, DFT2 over b
         dft4<INV>(v[0], v[2], v[4], v[6]);
         dft4<INV>(v[1], v[3], v[5], v[7]);
-        constexpr float H = 0.70710678118654752f;
-        v[3] = cmulf2(v[3], INV ? make_float2(H, H) : make_float2(H, -H));
+        constexpr double H = 0.70710678118654752;
+        v[3] = cmul(v[3], INV ? cmake<C>(H, H) : cmake<C>(H, -H));
         v[5] = rot<INV>(v[5]);
-        v[7] = cmulf2(v[7], INV ? make_float2(-H, H) : make_float2(-H, -H));
-        float2 o[8];
+        v[7] = cmul(v[7], INV ? cmake<C>(-H, H) : cmake<C>(-H, -H));
+        C o[8];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            float2 a = v[2 * c], b = v[2 * c + 1];
+            C a = v[2 * c], b = v[2 * c + 1];
             dft2<INV>(a, b);
             o[c] = a;
             o[c + 4] = b;
@@ -109,11 +134,11 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 #pragma unroll
         for (int b = 1; b < 4; ++b)
 #pragma unroll
-            for (int c = 1; c < 4; ++c) v[b + 4 * c] = cmulf2(v[b + 4 * c], w16(b * c, INV));
-        float2 o[16];
+            for (int c = 1; c < 4; ++c) v[b + 4 * c] = cmul(v[b + 4 * c], w16<C>(b * c, INV));
+        C o[16];
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            float2 a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
+            C a0 = v[4 * c], a1 = v[4 * c + 1], a2 = v[4 * c + 2], a3 = v[4 * c + 3];
             dft4<INV>(a0, a1, a2, a3);
             o[c] = a0;
             o[c + 4] = a1;
@@ -127,32 +152,35 @@ __device__ __forceinline__ void dft(float2 (&v)[R]) {
 
 // Twiddle tables in LDS, two levels each: e^{-2 pi i t / N} =
 // nlo[t & 63] nhi[t >> 6] (t < N) and e^{-2 pi i t / N_w} = wlo[t & 255]
-// whi[t >> 8] (t < N_w); every entry is rounded once from FP64, so a twiddle
-// carries about 1.5 ulp.
+// whi[t >> 8] (t < N_w); every entry is rounded once from FP64, so an FP32
+// twiddle carries about 1.5 ulp.
+template <typename C>
 struct Twiddles {
-    const float2 *nlo, *nhi, *wlo, *whi;
-    __device__ __forceinline__ float2 n(const int t) const { return cmulf2(nhi[t >> 6], nlo[t & 63]); }
-    __device__ __forceinline__ float2 w(const int t) const { return cmulf2(whi[t >> 8], wlo[t & 255]); }
+    const C *nlo, *nhi, *wlo, *whi;
+    __device__ __forceinline__ C n(const int t) const { return cmul(nhi[t >> 6], nlo[t & 63]); }
+    __device__ __forceinline__ C w(const int t) const { return cmul(whi[t >> 8], wlo[t & 255]); }
 };
 // Host-side layout of the table array: [64 | N/64 | 256 | N_w/256] entries.
-template <int N>
-__device__ __forceinline__ Twiddles load_twiddles(float2 *dst, const float2 *__restrict__ src,
-                                                  int nw) {
+template <int N, typename C>
+__device__ __forceinline__ Twiddles<C> load_twiddles(C *dst, const C *__restrict__ src, int nw) {
     const int total = 64 + N / 64 + 256 + nw / 256;
     for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = src[i];
-    return Twiddles{dst, dst + 64, dst + 64 + N / 64, dst + 64 + N / 64 + 256};
+    return Twiddles<C>{dst, dst + 64, dst + 64 + N / 64, dst + 64 + N / 64 + 256};
 }
 
 // One Stockham pass of radix R with span Ns, in place on the group's LDS
-// buffer sh (all threads of the workgroup reach both barriers).
-template <int N, int R, int Ns, bool INV>
-__device__ __forceinline__ void stockham_pass(float2 *sh, const Twiddles &tw, int gt) {
+// buffer sh, by T threads (gt < T; all threads of the workgroup reach both
+// barriers).  FP32 takes every twiddle from the tables; FP64 takes the pass
+// twiddle w from them and forms w^2 .. w^{R-1} by products (a few FP64 ulps),
+// which keeps the registers of fifteen table products free.
+template <int N, int R, int Ns, bool INV, int T, typename C>
+__device__ __forceinline__ void stockham_pass(C *sh, const Twiddles<C> &tw, int gt) {
     constexpr int NB = N / R;
-    constexpr int PER = (NB + 255) / 256;
-    float2 v[PER][R];
+    constexpr int PER = (NB + T - 1) / T;
+    C v[PER][R];
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int j = gt + 256 * b;
+        const int j = gt + T * b;
         if (j < NB)
 #pragma unroll
             for (int r = 0; r < R; ++r) v[b][r] = sh[phys(j + r * NB)];
@@ -160,14 +188,24 @@ __device__ __forceinline__ void stockham_pass(float2 *sh, const Twiddles &tw, in
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
-        const int j = gt + 256 * b;
+        const int j = gt + T * b;
         if (j < NB) {
             const int jm = j % Ns;
             if constexpr (Ns > 1) {
+                if constexpr (std::is_same<C, double2>::value) {
+                    const C w1 = tw.n(jm * (N / (Ns * R)));
+                    C w = w1;
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    const float2 t = tw.n(jm * r * (N / (Ns * R)));
-                    v[b][r] = cmulf2(v[b][r], INV ? conjf2(t) : t);
+                    for (int r = 1; r < R; ++r) {
+                        v[b][r] = cmul(v[b][r], INV ? cconj(w) : w);
+                        if (r + 1 < R) w = cmul(w, w1);
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 1; r < R; ++r) {
+                        const C t = tw.n(jm * r * (N / (Ns * R)));
+                        v[b][r] = cmul(v[b][r], INV ? cconj(t) : t);
+                    }
                 }
             }
             dft<R, INV>(v[b]);
@@ -179,21 +217,21 @@ __device__ __forceinline__ void stockham_pass(float2 *sh, const Twiddles &tw, in
     __syncthreads();
 }
 
-// All passes of an N-point transform on sh (natural order in and out):
-// radix 16 while a factor 16 is left, then the remaining 2, 4 or 8.
-template <int N, int Ns, bool INV>
-__device__ __forceinline__ void fft_passes(float2 *sh, const Twiddles &tw, int gt) {
+// All passes of an N-point transform on sh (natural order in and out) by T
+// threads: radix 16 while a factor 16 is left, then the remaining 2, 4 or 8.
+template <int N, int Ns, bool INV, int T, typename C>
+__device__ __forceinline__ void fft_passes(C *sh, const Twiddles<C> &tw, int gt) {
     if constexpr (Ns < N) {
         constexpr int rest = N / Ns;
         constexpr int R = rest >= 16 ? 16 : rest;
-        stockham_pass<N, R, Ns, INV>(sh, tw, gt);
-        fft_passes<N, Ns * R, INV>(sh, tw, gt);
+        stockham_pass<N, R, Ns, INV, T>(sh, tw, gt);
+        fft_passes<N, Ns * R, INV, T>(sh, tw, gt);
     }
 }
-template <int N, bool INV>
-__device__ __forceinline__ void fft_lds(float2 *sh, const Twiddles &tw, int gt) {
+template <int N, bool INV, int T = 256, typename C>
+__device__ __forceinline__ void fft_lds(C *sh, const Twiddles<C> &tw, int gt) {
     static_assert(N >= 64 && N <= 8192 && (N & (N - 1)) == 0, "N: power of two in [64, 8192]");
-    fft_passes<N, 1, INV>(sh, tw, gt);
+    fft_passes<N, 1, INV, T>(sh, tw, gt);
 }
 
 __device__ __forceinline__ int signed_freq(const int k, const int N) { return k < N / 2 ? k : k - N; }
@@ -255,7 +293,7 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
     const int p = blockIdx.x;
     const int tid = threadIdx.x, g = tid >> 8, gt = tid & 255;
     const int nw = U * N;
-    const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
+    const Twiddles<float2> tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
     const float2 wp = wfac[p];
     const float2 *y = Y + (size_t)p * N;
     float2 *row = prof + (size_t)p * ld;
@@ -296,7 +334,7 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
                     for (int r = 0; r < 16; ++r) sh[phys(16 * j + r)] = v[r];
                 }
                 __syncthreads();
-                fft_passes<N, 16, true>(sh, tw, gt);
+                fft_passes<N, 16, true, 256>(sh, tw, gt);
             } else {
 #pragma unroll
                 for (int i = 0; i < NI; ++i) {
@@ -342,36 +380,57 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
     }
 }
 
+// Thread groups of the forward: one transform per group of T threads, one or
+// two groups per workgroup as for Batch.  T = 256, and in FP64 max(256, N / 16)
+// (at most one radix-16 butterfly per thread and pass, which keeps the FP64
+// registers of N = 8192 in bounds).
+template <int N, int Q, typename C>
+struct DownBatch {
+    static constexpr bool kF64 = std::is_same<C, double2>::value;
+    static constexpr int T = kF64 && N / 16 > 256 ? N / 16 : 256;
+    static constexpr int G = Q >= 2 ? 2 : 1;
+    static constexpr int PPG = Q / G;
+    static constexpr int NT = T * G;
+};
+
 // Forward: Y[p][dst(k)] = W[k] wfac[p] sum_n prof[p][n] e^{-2 pi i s_k n / N_w}
 // (row stride nw = U N), k over the N band bins.  With `spans`, row p holds
 // data only in the spans[p].y bins from spans[p].x on (wrapping at N_w); the
 // other bins are taken as zero without being read; a count of -1 marks a
 // poisoned row, whose output is NaN.
-template <int N, int Q>
-__global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__restrict__ prof,
-                                                                const int2 *__restrict__ spans,
-                                                                const float2 *__restrict__ W,
-                                                                const float2 *__restrict__ wfac,
-                                                                int U, int centred,
-                                                                const float2 *__restrict__ tables,
-                                                                float2 *__restrict__ Y) {
-    using Bt = Batch<Q>;
-    extern __shared__ float2 lds[];
+//
+// C is the arithmetic of the transforms and of the sum over phases (LDS
+// buffers, twiddles, accumulators): float2 or double2; the profile is read and
+// the band written in FP32 either way.  Every phase transform carries the
+// whole row's energy, most of which lies outside the band and cancels in the
+// sum over phases.  For a smooth scene the band keeps a small fraction of it
+// (about 1/20 of the norm in the C4 looks that cancel most), and FP32
+// rounding inside the transforms, relative to the whole energy, is amplified
+// as much: about 8e-6 of the band there.  In FP64 what is left is the FP32
+// rounding of the profile itself.
+template <int N, int Q, typename C>
+__global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
+    const float2 *__restrict__ prof, const int2 *__restrict__ spans, const float2 *__restrict__ W,
+    const float2 *__restrict__ wfac, int U, int centred, const C *__restrict__ tables,
+    float2 *__restrict__ Y) {
+    using Bt = DownBatch<N, Q, C>;
+    extern __shared__ double2 lds64[];
+    C *lds = reinterpret_cast<C *>(lds64);
     constexpr int SZ = N + N / 16;
     constexpr int PER = (N + Bt::NT - 1) / Bt::NT;
     const int p = blockIdx.x;
-    const int tid = threadIdx.x, g = tid >> 8, gt = tid & 255;
+    const int tid = threadIdx.x, g = tid / Bt::T, gt = tid % Bt::T;
     const int nw = U * N;
-    const Twiddles tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
+    const Twiddles<C> tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
     const float2 *row = prof + (size_t)p * nw;
     const int2 sp = spans ? spans[p] : make_int2(0, nw);
     if (sp.y < 0) {  // poisoned row (see the merge)
         for (int k = tid; k < N; k += Bt::NT) Y[(size_t)p * N + k] = make_float2(NAN, NAN);
         return;
     }
-    float2 acc[PER];
+    C acc[PER];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int q = 0; q < PER; ++q) acc[q] = cmake<C>(0.0, 0.0);
     for (int r0 = 0; r0 < U; r0 += Q) {
 #pragma unroll 2
         for (int m = tid; m < N; m += Bt::NT) {
@@ -395,39 +454,69 @@ __global__ __launch_bounds__(Batch<Q>::NT) void fft_down_kernel(const float2 *__
                 for (int q = 0; q < Q; ++q) v[q] = make_float2(0.f, 0.f);
             }
 #pragma unroll
-            for (int q = 0; q < Q; ++q) lds[q * SZ + phys(m)] = v[q];
+            for (int q = 0; q < Q; ++q) lds[q * SZ + phys(m)] = cmake<C>(v[q].x, v[q].y);
         }
         __syncthreads();
 #pragma unroll 1
-        for (int pp = 0; pp < Bt::PPG; ++pp) fft_lds<N, false>(lds + (g + Bt::G * pp) * SZ, tw, gt);
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int k = tid + Bt::NT * q;
-            if (k >= N) break;
-            const int s = signed_freq(k, N);
+        for (int pp = 0; pp < Bt::PPG; ++pp)
+            fft_lds<N, false, Bt::T>(lds + (g + Bt::G * pp) * SZ, tw, gt);
+        if constexpr (Bt::kF64) {
 #pragma unroll
             for (int qq = 0; qq < Q; ++qq) {
-                int t = (s * (r0 + qq)) % nw;
-                t = t < 0 ? t + nw : t;
-                acc[q] = addf2(acc[q], cmulf2(lds[qq * SZ + phys(k)], tw.w(t)));
+                const int r = r0 + qq;
+                // e^{-2 pi i s_k r / N_w} for k = tid + NT q: from the tables at
+                // the first k of each half of the band (s_k = k, then k - N), and
+                // from there on times e^{-2 pi i NT r / N_w}
+                const C step = tw.w((Bt::NT * r) % nw);
+                C w = cmake<C>(1.0, 0.0);
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const int k = tid + Bt::NT * q;
+                    if (k >= N) break;
+                    if (q == 0 || q == PER / 2) {
+                        int t = (signed_freq(k, N) * r) % nw;
+                        t = t < 0 ? t + nw : t;
+                        w = tw.w(t);
+                    } else {
+                        w = cmul(w, step);
+                    }
+                    acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(k)], w));
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const int k = tid + Bt::NT * q;
+                if (k >= N) break;
+                const int s = signed_freq(k, N);
+#pragma unroll
+                for (int qq = 0; qq < Q; ++qq) {
+                    int t = (s * (r0 + qq)) % nw;
+                    t = t < 0 ? t + nw : t;
+                    acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(k)], tw.w(t)));
+                }
             }
         }
         __syncthreads();
     }
-    const float2 wp = wfac[p];
+    const C wp = cmake<C>(wfac[p].x, wfac[p].y);
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int k = tid + Bt::NT * q;
         if (k >= N) break;
         const int dst = centred ? (k + N / 2) % N : k;
-        Y[(size_t)p * N + dst] = cmulf2(cmulf2(W[k], acc[q]), wp);
+        const C y = cmul(cmul(cmake<C>(W[k].x, W[k].y), acc[q]), wp);
+        Y[(size_t)p * N + dst] = make_float2((float)y.x, (float)y.y);
     }
 }
 
-// LDS: Q phase buffers, then the twiddle tables.
-template <int N, int Q>
-size_t lds_bytes(int U) {
-    return sizeof(float2) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + (size_t)U * N / 256);
+// LDS: Q phase buffers, then the twiddle tables (elements of `elem` bytes:
+// float2 for the adjoint, double2 for the forward).
+constexpr size_t kMaxLds = 160 * 1024;  // LDS of one gfx950 workgroup
+
+#define SARHIP_FFT_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
+size_t fused_lds_bytes(int N, int U, int Q, size_t elem) {
+    return elem * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + (size_t)U * N / 256);
 }
 
 // ---- 2D spectral filter of the learned preconditioner ----------------------
@@ -598,7 +687,7 @@ hipError_t spectral_filter(const float2 *in, const float2 *din, int B, int n, co
 template <int N, int Q>
 hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac, int U,
               int ld, int P, int centred, const float2 *tables, float2 *prof, hipStream_t s) {
-    const size_t lds = lds_bytes<N, Q>(U);
+    const size_t lds = fused_lds_bytes(N, U, Q, sizeof(float2));
     hipError_t e = hipFuncSetAttribute((const void *)fft_up_kernel<N, Q>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
@@ -607,15 +696,45 @@ hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const flo
     return hipGetLastError();
 }
 
-template <int N, int Q>
+template <int N, int Q, typename C>
 hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac, int U,
-                int P, int centred, const float2 *tables, float2 *Y, hipStream_t s) {
-    const size_t lds = lds_bytes<N, Q>(U);
-    hipError_t e = hipFuncSetAttribute((const void *)fft_down_kernel<N, Q>,
+                int P, int centred, const C *tables, float2 *Y, hipStream_t s) {
+    const size_t lds = fused_lds_bytes(N, U, Q, sizeof(C));
+    hipError_t e = hipFuncSetAttribute((const void *)fft_down_kernel<N, Q, C>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    fft_down_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(prof, spans, W, wfac, U, centred, tables, Y);
+    fft_down_kernel<N, Q, C><<<P, DownBatch<N, Q, C>::NT, lds, s>>>(prof, spans, W, wfac, U,
+                                                                     centred, tables, Y);
     return hipGetLastError();
+}
+
+// The forward's instantiations whose buffers fit the LDS for every admitted
+// N_w (tables of at most 64 + N/64 + 256 + 512 entries); the others are never
+// selected by fused_fft_phases and are not built.
+template <int N, int Q, typename C>
+hipError_t down_fit(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
+                    int U, int P, int centred, const C *tables, float2 *Y, hipStream_t s) {
+    constexpr size_t need = sizeof(C) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + 512);
+    if constexpr (need <= kMaxLds)
+        return down<N, Q, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s);
+    else
+        return hipErrorInvalidValue;
+}
+
+template <typename C>
+hipError_t down_n(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
+                  int N, int U, int P, int Q, int centred, const C *tables, float2 *Y,
+                  hipStream_t s) {
+#define DOWN_CASE(n)                                                                             \
+    if (N == n) {                                                                                \
+        if (Q == 8) return down_fit<n, 8, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        if (Q == 4) return down_fit<n, 4, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        if (Q == 2) return down_fit<n, 2, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
+        return down_fit<n, 1, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s);             \
+    }
+    SARHIP_FFT_SIZES(DOWN_CASE)
+#undef DOWN_CASE
+    return hipErrorInvalidValue;
 }
 
 }  // namespace
@@ -628,31 +747,29 @@ hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const fl
 // dominate and occupancy wins: cap 1 (0.19 / 0.21 ms against 0.24 / 0.23 ms
 // with the dense caps, spans of ~9% of the row).  SARHIP_FFT_PHASES
 // overrides every cap.
-int fused_fft_phases(int N, int U, bool up, bool sparse) {
-    const size_t per = sizeof(float2) * (size_t)(N + N / 16);
-    const size_t tab = sizeof(float2) * (64 + N / 64 + 256 + (size_t)U * N / 256);
+int fused_fft_phases(int N, int U, bool up, bool sparse, bool fp64) {
+    const size_t elem = fp64 ? sizeof(double2) : sizeof(float2);
     int cap = sparse ? 1 : up ? 8 : 2;
     if (const char *e = std::getenv("SARHIP_FFT_PHASES")) cap = std::atoi(e);
     for (int Q : {8, 4, 2}) {
-        if (Q <= cap && U % Q == 0 && Q * per + tab <= 160 * 1024) return Q;
+        if (Q <= cap && U % Q == 0 && fused_lds_bytes(N, U, Q, elem) <= kMaxLds) return Q;
     }
     return 1;
 }
 
 // N a power of two in [64, 8192], N_w = U N a multiple of 256 up to 2^17
-// (the two-level twiddle tables stay small).
+// (the two-level twiddle tables stay small; one FP64 phase buffer and the
+// tables then fit the LDS: 151 KiB at N = 8192, N_w = 2^17).
 bool fused_fft_supported(int N, int U) {
     const long long nw = (long long)U * N;
     return N >= 64 && N <= 8192 && (N & (N - 1)) == 0 && U >= 1 && nw % 256 == 0 &&
-           nw <= (1 << 17);
+           nw <= (1 << 17) && fused_lds_bytes(N, U, 1, sizeof(double2)) <= kMaxLds;
 }
-
-#define SARHIP_FFT_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
                          float2 *prof, bool sparse, hipStream_t s) {
-    const int Q = fused_fft_phases(N, U, true, sparse);
+    const int Q = fused_fft_phases(N, U, true, sparse, false);
 #define UP_CASE(n)                                                                         \
     if (N == n) {                                                                          \
         if (Q == 8) return up<n, 8>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
@@ -666,19 +783,12 @@ hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj
 }
 
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
-                           int N, int U, int P, int centred, const float2 *tables, float2 *Y,
-                           bool sparse, hipStream_t s) {
-    const int Q = fused_fft_phases(N, U, false, sparse);
-#define DOWN_CASE(n)                                                                   \
-    if (N == n) {                                                                      \
-        if (Q == 8) return down<n, 8>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        if (Q == 4) return down<n, 4>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        if (Q == 2) return down<n, 2>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        return down<n, 1>(prof, spans, W, wfac, U, P, centred, tables, Y, s);             \
-    }
-    SARHIP_FFT_SIZES(DOWN_CASE)
-#undef DOWN_CASE
-    return hipErrorInvalidValue;
+                           int N, int U, int P, int centred, const float2 *tables,
+                           const double2 *tables64, float2 *Y, bool sparse, hipStream_t s) {
+    const bool fp64 = tables64 != nullptr;
+    const int Q = fused_fft_phases(N, U, false, sparse, fp64);
+    if (fp64) return down_n<double2>(prof, spans, W, wfac, N, U, P, Q, centred, tables64, Y, s);
+    return down_n<float2>(prof, spans, W, wfac, N, U, P, Q, centred, tables, Y, s);
 }
 
 bool spectral_filter_supported(int n, int M) {

@@ -148,6 +148,17 @@ __device__ __forceinline__ void phasor_fract(const double theta, float *s, float
     *s = __builtin_amdgcn_sinf(r);
     *c = __builtin_amdgcn_cosf(r);
 }
+// As phasor_fract, with the argument's FP32 rounding residual d (radians)
+// carried by a first-order correction: sin(r + d) = s + d c, cos(r + d) = c - d s
+// (|d| <= pi 2^-24, so the neglected d^2 / 2 is below 2^-45).
+__device__ __forceinline__ void phasor_fract_fine(const double theta, float *s, float *c) {
+    const double f = __builtin_amdgcn_fract(theta);
+    const float r = (float)f;
+    const float d = (float)(f - (double)r) * 6.28318530717958648f;
+    const float s0 = __builtin_amdgcn_sinf(r), c0 = __builtin_amdgcn_cosf(r);
+    *s = fmaf(d, c0, s0);
+    *c = fmaf(-d, s0, c0);
+}
 __device__ __forceinline__ void phasor_rev(const double theta, float *s, float *c) {
     const float r = (float)(theta - rint(theta));
     *s = __builtin_amdgcn_sinf(r);

@@ -572,21 +572,22 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 //
 // Window bins accumulate in 32-bit fixed point with LDS integer atomics: on
 // gfx950 ds_add_u32 runs ~40x faster than ds_add_f32.  The scale is
-// 2^31 / (1.001 sum_region(|Re a| + |Im a|)): no bin sum can exceed the
-// region's L1 norm (interpolation weights are at most 1) and the conversion
-// to integers truncates toward zero, which never increases a magnitude, so
-// every sum stays inside int32; FP32 products carry ~6e-8 relative error and
-// the truncation adds less than 2^-31 of the norm per contribution.  The
+// 2^31 / (1.001 B) (FixScale) for the region's bound B on every bin
+// sum (region_l1_kernel), and the conversion to integers truncates toward
+// zero, which never increases a magnitude, so every sum stays inside int32.
+// FP32 pre-scaling and products carry ~6e-8 relative error and the
+// truncation less than one unit per contribution.  The
 // integer parts of the two linear-interpolation shares sum to the truncated
 // total.  A region whose amplitudes are not finite (and any contribution
 // outside its window, which the window bounds rule out) cannot be windowed:
 // with cleared rows it goes to float global atomics, on the span path the
 // row is poisoned instead.
 //
-// Windows are converted back to float and written whole (zeros included) to
-// `partial` with their base bins and max |component|, then merged per pulse
-// by fp_reduce_kernel: no global atomics on the main path and a result that
-// does not depend on scheduling.
+// Windows are written whole (zeros included) to `partial` as their integer
+// sums, with their base bins and max |component| (in their units), then
+// merged per pulse by fp_reduce_kernel in 64-bit integers: no global atomics
+// on the main path, no rounding between the windows and the row, and a result
+// that does not depend on scheduling.
 // ---------------------------------------------------------------------------
 // Toward zero (one v_cvt_i32_f32).  Its error is odd in v, so bin sums of
 // contributions with mixed signs carry no bias; rounding halves always up
@@ -596,9 +597,12 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 __device__ __forceinline__ int fix_tz(const float v) { return (int)v; }
 // Fixed-point scale 2^31 / (1.001 b) for a bound b on every bin sum, held as
 // m 2^e with m in (1, 2]: values are scaled by ldexp(v m, e) and sums taken
-// back by ldexp(s / m, -e).  2^31 / b itself leaves the FP32 range for b below
-// about 6e-30 (and 1 / that scale for b below 1e-29 is subnormal); the split
-// form stays exact there, so the forward stays linear for tiny inputs.
+// back as s (1 / m) 2^-e, 1 / m = mm the mantissa of 1.001 b, in [1/2, 1).
+// 2^31 / b itself leaves the FP32 range for b below about 6e-30 (and 1 / that
+// scale for b below 1e-29 is subnormal); the split form stays exact there, so
+// the forward stays linear for tiny inputs.  (A power-of-two scale would make
+// the pre-scaling exact but leave up to one bit of the int32 range unused,
+// doubling the truncation unit, the larger error of the two.)
 struct FixScale {
     float m, inv_m;
     int e;
@@ -922,7 +926,11 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     // forward bin rule round(u) for both nearest modes (CPP:289)
                     fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
                 }
-                phasor_fract(th, &sn, &cs);
+                // exact ranges: the phase argument's FP32 rounding is corrected
+                if constexpr (PREC == kPrecFp64)
+                    phasor_fract_fine(th, &sn, &cs);
+                else
+                    phasor_fract(th, &sn, &cs);
                 // contribution a e^{i theta} (times `scale` when pre-scaled)
                 const float cr = fmaf(cs, a.x, -(sn * a.y));
                 const float ci = fmaf(cs, a.y, sn * a.x);
@@ -1034,55 +1042,73 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
         }
     }
     __syncthreads();
-    // Partials and per-window maxima: wave w converts the windows of pulses
-    // w, w + NT/64, ... (coalesced rows) and reduces their maxima in registers.
+    // Partials and per-window maxima: wave w writes the integer windows of
+    // pulses w, w + NT/64, ... (coalesced rows) and reduces their maxima in
+    // registers (|sum| < 2^31, so the magnitudes do not overflow); the maximum
+    // is kept in the window's own units, 0 for a window that is all zero.
     const int NR = k.num_regions;
     const int lane = tid & 63, wave = tid >> 6;
     for (int qq = wave; qq < np; qq += NT / 64) {
-        float mx = 0.f;
-        float2 *dst = k.partial + ((size_t)(j0 + qq) * NR + region) * win;
+        int mx = 0;
+        int2 *dst = k.partial + ((size_t)(j0 + qq) * NR + region) * win;
         for (int l = lane; l < win; l += 64) {
             const unsigned long long w = wsum[qq * wst + l];
-            const int2 v = make_int2((int)(unsigned)(w & 0xffffffffull), (int)(unsigned)(w >> 32));
-            const float2 f = scaled ? make_float2(fix_undo(fs, (float)v.x), fix_undo(fs, (float)v.y))
-                                    : make_float2(0.f, 0.f);
-            dst[l] = f;
-            mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
+            const int2 v = scaled ? make_int2((int)(unsigned)(w & 0xffffffffull),
+                                              (int)(unsigned)(w >> 32))
+                                  : make_int2(0, 0);
+            dst[l] = v;
+            mx = max(mx, max(abs(v.x), abs(v.y)));
         }
-        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-        if (lane == 0) k.wmax[(size_t)(j0 + qq) * NR + region] = mx;
+        for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
+        if (lane == 0) k.wmax[(size_t)(j0 + qq) * NR + region] = (float)mx;
     }
     if (tid < np) k.wbases[(size_t)(j0 + tid) * NR + region] = wb;
 }
 
-// One workgroup per valid pulse: merge the pulse's window partials into its
-// profile row.  Window bases and maxima are staged in LDS; the covered span of
-// bins is accumulated in LDS chunks while the partials stream through
-// coalesced loads: each wave takes whole windows (a window's base is then
-// uniform over the wave), kRedWin windows x kRedSlices 64-bin slices per
-// batch, so that enough bytes are in flight to approach HBM bandwidth.  The
-// chunk accumulates in 32-bit fixed point with LDS integer atomics, scaled by
-// 2^31 / (1.001 x the sum over windows of the window's max |component|), a
-// bound on any bin sum of the row (rounding adds at most half a unit per
-// window, far inside the 0.1% margin); if that bound is not finite the chunk
-// accumulates in float.  The chunk is then stored into the row (span path)
-// or added to it (rows cleared before the scatter).
+// One workgroup per valid pulse: merge the pulse's integer window partials
+// into its profile row, with no atomics on the sums.  The covered span of bins
+// is cut into 64-bin blocks, and every non-zero window is listed under each
+// block it meets (a counting sort in LDS).  Wave w takes blocks w, w + 8, ...:
+// lane l sums bin 64 b + l over the block's windows in 64-bit integer
+// registers, a window's part of the block being one coalesced load, kRedBatch
+// windows per batch; every partial is read once.
+//
+// Sums are at the unit 2^-E, E = e_min + H, with e_min the coarsest scale
+// exponent among the row's non-zero windows and H = clz(NR) - 1: a window sum
+// s (|s| < 2^31 units mm 2^-e, mm = M 2^-24 with the integer M < 2^24) enters
+// as s M 2^(E - e - 24), formed in integers (a 64-bit product, then a shift;
+// right shifts round), below 2^(31 + H), so a sum over the NR windows stays
+// below 2^62; the rounding is at most 2^-(32 + H) of the coarsest window's
+// bound.  Each bin is converted once (the row's only further rounding) and
+// stored into the row (span path) or added to it (rows cleared before the
+// scatter).  The span is merged in passes of at most min(N_w, 64 kRedBlocks)
+// bins, so that no two bins of one pass share a row bin when it is longer
+// than the row.
 constexpr int kRedThreads = 512;
-constexpr int kRedWin = 4;     // windows per wave per batch
-constexpr int kRedSlices = 2;  // 64-bin slices per window per batch
-__global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k, int chunk) {
-    // [chunk] re words, [chunk] im words (separate planes: consecutive bins
-    // fall in consecutive banks), then [NR] bases (double)
-    extern __shared__ int2 acc[];
+constexpr int kRedBatch = 4;                   // windows per batch
+constexpr int kRedBlocks = 4 * kRedThreads;    // 64-bin blocks per pass
+__device__ __forceinline__ long long red_at_unit(const int x, const int2 f) {
+    const long long v = (long long)x * (long long)f.x;
+    if (f.y >= 0) return v << f.y;
+    if (f.y <= -63) return 0;
+    return (v + (1ll << (-f.y - 1))) >> -f.y;
+}
+__global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k) {
+    // [NR] window bases (double, NaN for a window that is all zero), [NR]
+    // window factors {M, shift}, then the block lists (window indices)
+    extern __shared__ double sb[];
+    __shared__ int cnt[kRedBlocks + 1];  // counts, then list offsets
+    __shared__ int cur[kRedBlocks];      // fill cursors
     __shared__ double red_lo[kRedThreads], red_hi[kRedThreads];
-    __shared__ float red_l1[kRedThreads];
+    __shared__ int red_e[kRedThreads];
     const int j = blockIdx.x;
     const int p = k.rows[j];
     const int NR = k.num_regions, win = k.win;
-    double *sb = (double *)(acc + chunk);
+    int2 *su = (int2 *)(sb + NR);
+    int *list = (int *)(su + NR);
     const double *b = k.wbases + (size_t)j * NR;
     const float *wm = k.wmax + (size_t)j * NR;
-    const float2 *part = k.partial + (size_t)j * NR * win;
+    const int2 *part = k.partial + (size_t)j * NR * win;
     const int tid = (int)threadIdx.x;
     // a row the scatter poisoned: no span, NaN output row (flag cleared here)
     __shared__ int s_poison;
@@ -1096,111 +1122,151 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
         return;
     }
     double lo = INFINITY, hi = -INFINITY;
-    float l1 = 0.f;
+    int emin = INT_MAX;
     for (int t = tid; t < NR; t += kRedThreads) {
         const double bt = b[t];
-        sb[t] = bt;
-        lo = fmin(lo, bt);
-        hi = fmax(hi, bt);
-        l1 += wm[t];
+        const bool live = wm[t] > 0.f;  // a window with a non-zero sum
+        sb[t] = live ? bt : NAN;
+        if (live) {
+            lo = fmin(lo, bt);
+            hi = fmax(hi, bt);
+            emin = min(emin, fix_scale(k.region_l1[t]).e);
+        }
     }
     red_lo[tid] = lo;
     red_hi[tid] = hi;
-    red_l1[tid] = l1;
+    red_e[tid] = emin;
     __syncthreads();
     for (int s = kRedThreads / 2; s > 0; s >>= 1) {
         if (tid < s) {
             red_lo[tid] = fmin(red_lo[tid], red_lo[tid + s]);
             red_hi[tid] = fmax(red_hi[tid], red_hi[tid + s]);
-            red_l1[tid] += red_l1[tid + s];
+            red_e[tid] = min(red_e[tid], red_e[tid + s]);
         }
         __syncthreads();
     }
     lo = red_lo[0];
     hi = red_hi[0] + (double)win;
-    l1 = red_l1[0];
-    const bool fixed = fix_bound_ok(l1);
-    FixScale fs{1.f, 1.f, 0};
-    if (fixed && l1 > 0.f) fs = fix_scale(l1);
     // With fwd_spans the covered bins [lo, hi) are written (stored) once each,
     // and their span is recorded for the spectral stage; the rest of the row
     // is not touched (a span longer than the row wraps onto itself: the row is
-    // zeroed first and the chunks add into it).  Otherwise the row was cleared
-    // before the scatter, which may have added global atomics, and the chunks
+    // zeroed first and the passes add into it).  Otherwise the row was cleared
+    // before the scatter, which may have added global atomics, and the passes
     // add into it.
     const bool store = k.fwd_spans != 0;
-    const bool empty = !(hi > lo) || (fixed && l1 == 0.f);  // every window zero
+    const bool empty = !(hi > lo);  // every window zero
     const bool full = !empty && hi - lo > (double)k.nw;
     if (tid == 0)
         k.fspan[p] = empty ? make_int2(0, 0)
                            : full ? make_int2(0, k.nw) : make_int2(bin_index(k, lo, 1), (int)(hi - lo));
     if (empty) return;
-    int *acc_re = (int *)acc, *acc_im = acc_re + chunk;
-    float *facc_re = (float *)acc_re, *facc_im = (float *)acc_im;
+    const int E = red_e[0] + __clz(NR) - 1;  // the accumulation unit 2^-E
+    const double unit = ldexp(1.0, -E);
+    for (int t = tid; t < NR; t += kRedThreads) {
+        int2 f = make_int2(0, 0);
+        if (!isnan(sb[t])) {
+            const FixScale fs = fix_scale(k.region_l1[t]);
+            f = make_int2((int)ldexpf(fs.inv_m, 24), E - fs.e - 24);
+        }
+        su[t] = f;
+    }
     float2 *row = k.prof + (size_t)p * k.nw;
     if (full && store) {
         for (int e = tid; e < k.nw; e += kRedThreads) row[e] = make_float2(0.f, 0.f);
         __threadfence_block();
-        __syncthreads();
     }
-    // wave w merges windows w, w + 8, ...: a window's base is uniform over
-    // the wave and its bins are contiguous loads; kRedWin windows per batch
-    // keep several loads in flight per lane
+    __syncthreads();
+    const int S = (int)(hi - lo);  // bins of the span
+    const int W = min(k.nw, 64 * kRedBlocks);
     const int lane = tid & 63, wave = tid >> 6;
     constexpr int kWaves = kRedThreads / 64;
-    const int nl = (win + 63) / 64;  // 64-bin slices per window
-    for (double c0 = lo; c0 < hi; c0 += (double)chunk) {
-        for (int e = tid; e < chunk; e += kRedThreads) acc_re[e] = acc_im[e] = 0;
+    for (int c0 = 0; c0 < S; c0 += W) {
+        const int len = min(W, S - c0);
+        const int nb = (len + 63) / 64;
+        // blocks [first, last] of this pass that window t meets
+        auto blocks = [&](const int t, int &first, int &last) -> bool {
+            const double bt = sb[t];
+            if (isnan(bt)) return false;
+            const int r0 = (int)(bt - lo) - c0;  // the window's first bin in the pass
+            if (r0 >= len || r0 + win <= 0) return false;
+            first = max(r0, 0) >> 6;
+            last = min(r0 + win - 1, len - 1) >> 6;
+            return true;
+        };
+        for (int e = tid; e <= nb; e += kRedThreads) cnt[e] = 0;
         __syncthreads();
-        for (int t0 = wave; t0 < NR; t0 += kWaves * kRedWin) {
-            for (int s0 = 0; s0 < nl; s0 += kRedSlices) {
-                float2 v[kRedWin][kRedSlices];
-#pragma unroll
-                for (int w = 0; w < kRedWin; ++w) {
-                    const int t = t0 + kWaves * w;
-#pragma unroll
-                    for (int sl = 0; sl < kRedSlices; ++sl) {
-                        const int l = (s0 + sl) * 64 + lane;
-                        v[w][sl] = (t < NR && l < win) ? part[(size_t)t * win + l]
-                                                       : make_float2(0.f, 0.f);
-                    }
-                }
-#pragma unroll
-                for (int w = 0; w < kRedWin; ++w) {
-                    const int t = t0 + kWaves * w;
-                    if (t >= NR) break;
-                    const double base = sb[t] - c0;
-#pragma unroll
-                    for (int sl = 0; sl < kRedSlices; ++sl) {
-                        const int l = (s0 + sl) * 64 + lane;
-                        const double rel = base + (double)l;
-                        if (l < win && rel >= 0.0 && rel < (double)chunk) {
-                            const int r = (int)rel;
-                            if (fixed) {
-                                atomicAdd(&acc_re[r], (int)rintf(fix_apply(fs, v[w][sl].x)));
-                                atomicAdd(&acc_im[r], (int)rintf(fix_apply(fs, v[w][sl].y)));
-                            } else {
-                                atomicAdd(&facc_re[r], v[w][sl].x);
-                                atomicAdd(&facc_im[r], v[w][sl].y);
-                            }
-                        }
-                    }
-                }
-            }
+        for (int t = tid; t < NR; t += kRedThreads) {
+            int f, l;
+            if (blocks(t, f, l))
+                for (int q = f; q <= l; ++q) atomicAdd(&cnt[q], 1);
         }
         __syncthreads();
-        const int len = (int)fmin((double)chunk, hi - c0);
-        for (int e = tid; e < len; e += kRedThreads) {
-            const int n = bin_index(k, c0 + (double)e, 1);
-            float2 r = (full || !store) ? row[n] : make_float2(0.f, 0.f);
-            if (fixed) {
-                r.x += fix_undo(fs, (float)acc_re[e]);
-                r.y += fix_undo(fs, (float)acc_im[e]);
-            } else {
-                r.x += facc_re[e];
-                r.y += facc_im[e];
+        // exclusive scan of cnt[0, nb) (4 consecutive entries per thread), the
+        // total in cnt[nb]; the fill cursors start at the offsets
+        {
+            const int e0 = 4 * tid;
+            int v4[4], sum = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                v4[q] = e0 + q < nb ? cnt[e0 + q] : 0;
+                sum += v4[q];
             }
-            row[n] = r;
+            red_e[tid] = sum;
+            __syncthreads();
+            for (int off = 1; off < kRedThreads; off <<= 1) {
+                const int add = tid >= off ? red_e[tid - off] : 0;
+                __syncthreads();
+                red_e[tid] += add;
+                __syncthreads();
+            }
+            int run = red_e[tid] - sum;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (e0 + q < nb) {
+                    cnt[e0 + q] = run;
+                    cur[e0 + q] = run;
+                }
+                run += v4[q];
+            }
+            if (tid == kRedThreads - 1) cnt[nb] = red_e[tid];
+        }
+        __syncthreads();
+        for (int t = tid; t < NR; t += kRedThreads) {
+            int f, l;
+            if (blocks(t, f, l))
+                for (int q = f; q <= l; ++q) list[atomicAdd(&cur[q], 1)] = t;
+        }
+        __syncthreads();
+        for (int blk = wave; blk < nb; blk += kWaves) {
+            const int nrel = 64 * blk + lane;  // the lane's bin in the pass
+            const int beg = cnt[blk], end = cnt[blk + 1];
+            long long ar = 0, ai = 0;
+            for (int i = beg; i < end; i += kRedBatch) {
+                int2 v[kRedBatch], f[kRedBatch];
+#pragma unroll
+                for (int q = 0; q < kRedBatch; ++q) {
+                    v[q] = make_int2(0, 0);
+                    f[q] = make_int2(0, 0);
+                    if (i + q < end) {
+                        const int t = list[i + q];
+                        const int o = c0 + nrel - (int)(sb[t] - lo);
+                        f[q] = su[t];
+                        if (o >= 0 && o < win) v[q] = part[(size_t)t * win + o];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kRedBatch; ++q) {
+                    ar += red_at_unit(v[q].x, f[q]);
+                    ai += red_at_unit(v[q].y, f[q]);
+                }
+            }
+            if (nrel < len) {
+                const int n = bin_index(k, lo + (double)(c0 + nrel), 1);
+                float2 r = (full || !store) ? row[n] : make_float2(0.f, 0.f);
+                r.x += (float)((double)ar * unit);
+                r.y += (float)((double)ai * unit);
+                row[n] = r;
+            }
         }
         __threadfence_block();
         __syncthreads();
@@ -1479,9 +1545,20 @@ hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s) {
-    const size_t lds = sizeof(int2) * (size_t)chunk + sizeof(double) * (size_t)k.num_regions;
-    fp_reduce_kernel<<<k.Pv, kRedThreads, lds, s>>>(k, chunk);
+size_t fp_reduce_lds(int num_regions, int win) {
+    const size_t nr = (size_t)std::max(num_regions, 0);
+    const size_t per = sizeof(double) + sizeof(int2) + sizeof(int) * ((size_t)std::max(win - 1, 0) / 64 + 2);
+    return nr * per;
+}
+
+hipError_t launch_fp_reduce(const KernelParams &k, hipStream_t s) {
+    const size_t lds = fp_reduce_lds(k.num_regions, k.win);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void *)fp_reduce_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    fp_reduce_kernel<<<k.Pv, kRedThreads, lds, s>>>(k);
     return hipGetLastError();
 }
 

@@ -42,9 +42,9 @@ struct KernelParams {
     int fwd_groups;           // ceil(Pv / fwd_pulses)
     int win;                  // LDS window length per pulse (0 = no windows)
     int win_stride;           // odd per-pulse stride of the LDS window planes (>= win)
-    float2 *partial;          // [Pv][num_regions][win]
+    int2 *partial;            // [Pv][num_regions][win] integer window sums
     double *wbases;           // [Pv][num_regions]
-    float *wmax;              // [Pv][num_regions] max |component| of each window
+    float *wmax;              // [Pv][num_regions] max |component| of each window (its units)
     int2 *fspan;              // [P] covered bins of each forward row: {first bin, count}; count
                               // -1: the row is poisoned (a contribution could not be windowed)
     int *fpoison;             // [P] set by the scatter, read and cleared by the merge
@@ -129,19 +129,20 @@ hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, con
                                 float2 *out, hipStream_t s);
 // fused spectral stages (fft_kernels.hip): N-sample band <-> U N-sample rows
 bool fused_fft_supported(int N, int U);
-int fused_fft_phases(int N, int U, bool up, bool sparse);
+int fused_fft_phases(int N, int U, bool up, bool sparse, bool fp64);
 // tables: [64 | N/64 | 256 | U N/256] two-level twiddle tables,
 // e^{-2 pi i q / N}, e^{-2 pi i 64 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
+// (tables64: the same in FP64)
 // spans: per-row bins to store {first, count} (and their tail copies), or NULL for whole rows;
 // sparse: the spans cover a small part of the rows (picks the phase batching)
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
                          float2 *prof, bool sparse, hipStream_t s);
 // spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows;
-// sparse: as for launch_fft_up
+// sparse: as for launch_fft_up; tables64 non-NULL: transforms and phase sums in FP64
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
-                           int N, int U, int P, int centred, const float2 *tables, float2 *Y,
-                           bool sparse, hipStream_t s);
+                           int N, int U, int P, int centred, const float2 *tables,
+                           const double2 *tables64, float2 *Y, bool sparse, hipStream_t s);
 // spectral filter of the learned preconditioner (fft_kernels.hip): out =
 // dout . crop_n IFFT2_M(f / M^2 . FFT2_M(pad_M(conj(din) . in))), B images n x n,
 // filter f [M][M] float (ky, kx), work >= B n M complex64, din / dout optional
@@ -150,6 +151,8 @@ hipError_t launch_spectral_filter(const float2 *in, const float2 *din, int B, in
                                   const float *f, float2 *work, float2 *out, const float2 *dout,
                                   hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
-hipError_t launch_fp_reduce(const KernelParams &k, int chunk, hipStream_t s);
+// dynamic LDS bytes of the merge for the number of forward regions and the window length
+size_t fp_reduce_lds(int num_regions, int win);
+hipError_t launch_fp_reduce(const KernelParams &k, hipStream_t s);
 
 }  // namespace sarhip

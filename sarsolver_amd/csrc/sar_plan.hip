@@ -110,7 +110,7 @@ struct sarhip_plan_s {
     bool mono = false;
     bool planar = false;
     int Pv = 0;                  // valid pulses
-    int num_tiles = 0, win = 0, reduce_chunk = 0;
+    int num_tiles = 0, win = 0;
     int num_regions = 0, region_tiles = 1, fwd_pulses = 32, fwd_groups = 1;
     double series_err_bins = 0;  // bound on the series truncation term (bins)
     double bp_series_err_bins = 0;  // the same for the linear kernels' series order
@@ -128,13 +128,14 @@ struct sarhip_plan_s {
     int fused_U = 0;                                // > 0: fused spectral kernels, N_w = U N
     bool fft_sparse = false;  // adjoint spans cover under half of the valid rows' bins
     float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
+    double2 *d_fft_tables64 = nullptr;              // the same in FP64 (forward, exact ranges)
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     SeriesGather *d_series_g = nullptr;  // [num_tiles][Pv] gather records (see SeriesGather)
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
     float2 *d_amps_perm = nullptr;
-    float2 *d_partial = nullptr;
+    int2 *d_partial = nullptr;
     double *d_wbases = nullptr;
     float *d_wmax = nullptr;
     int2 *d_fspan = nullptr;  // [P] forward row spans written by the merge
@@ -173,6 +174,7 @@ struct sarhip_plan_s {
         dev_free(d_series_g);
         dev_free(d_srec);
         dev_free(d_fft_tables);
+        dev_free(d_fft_tables64);
         dev_free(d_W);
         dev_free(d_Wconj);
         dev_free(d_wfac_fwd);
@@ -800,6 +802,11 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         const int forced_g = env_int("SARHIP_FP_PULSES", 0);
         double best_cost = INFINITY;
         int best_r = 1, best_g = 32, best_w = 0;
+        // the merge's window tables beside its 26 KiB of static LDS
+        auto merge_fits = [&](int R, int w) {
+            const size_t nreg = (pl->num_tiles + (size_t)R - 1) / (size_t)R;
+            return nreg <= (size_t)INT_MAX && fp_reduce_lds((int)nreg, w) <= 128 * 1024;
+        };
         // 64 pulses per workgroup (512 threads, wave-uniform pixel streams)
         // whenever its windows fit in 53 KiB (three workgroups per CU; with
         // two, C2's 8-tile regions ran slower than 2-tile ones in 32 KiB);
@@ -815,13 +822,16 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                 // (dense-image forward against the FP64 oracle: 2.2e-7 with
                 // float atomics, 5e-7 / 9e-7 / 1.8e-6 / 3.6e-6 for regions of
                 // 1 / 2 / 4 / 8 tiles); regions stay within 2 tiles unless
-                // forced.
-                if (forced_r == 0 && R > 2) continue;
+                // forced or the merge's LDS tables need fewer regions.
+                if (forced_r == 0 && R > 2 && merge_fits(2, 0)) continue;
                 std::vector<double4> reg, hal;
                 if ((rc = tile_stats((size_t)R * kTilePixels, false, reg, hal))) return rc;
                 const double wd = window_bins(reg, hal, pc, rows, mono, pl->by_dx);
                 if (wd > 1e6) continue;
                 const int w = (int)wd;
+                // the merge stages every region's window base, scale and block
+                // lists in LDS
+                if (!merge_fits(R, w)) continue;
                 for (int G : {64, 32, 16}) {
                     if (forced_g > 0 && G != forced_g) continue;
                     if ((pass == 0) != (G == 64)) continue;
@@ -850,7 +860,6 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         pl->fwd_groups = (std::max(pl->Pv, 1) + best_g - 1) / best_g;
         pl->win = best_w;
         pl->num_regions = (int)regions.size();
-        pl->reduce_chunk = (int)std::min<size_t>((40 * 1024) / sizeof(int2), nw);
     }
 
     // Waveform and weights (complex64).
@@ -924,7 +933,7 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     }
     if (pl->win > 0)
         HIP_TRY(hipMalloc((void **)&pl->d_partial,
-                          sizeof(float2) * nrows * pl->num_regions * (size_t)pl->win));
+                          sizeof(int2) * nrows * pl->num_regions * (size_t)pl->win));
     if (need_series && pl->Pv > 0) {
         const size_t rec = mono ? sizeof(SeriesMono) : sizeof(SeriesBi);
         HIP_TRY(hipMalloc(&pl->d_series, rec * pl->num_tiles * (size_t)pl->Pv));
@@ -953,19 +962,26 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         const size_t U = nw / N;
         pl->fused_U = 0;
         if (U * N == nw && fused_fft_supported((int)N, (int)U) && env_int("SARHIP_FUSED_FFT", 1)) {
-            // two-level twiddle tables, each entry rounded once from FP64
-            std::vector<float2> tab;
+            // two-level twiddle tables in FP64, and each entry rounded once to FP32
+            std::vector<double2> tab64;
             auto put = [&](size_t count, double step, size_t len) {
                 for (size_t q = 0; q < count; ++q) {
                     const double a = -2.0 * M_PI * (double)(q * step) / (double)len;
-                    tab.push_back(make_float2((float)std::cos(a), (float)std::sin(a)));
+                    tab64.push_back(make_double2(std::cos(a), std::sin(a)));
                 }
             };
             put(64, 1, N);
             put(N / 64, 64, N);
             put(256, 1, nw);
             put(nw / 256, 256, nw);
+            std::vector<float2> tab;
+            for (const double2 &t : tab64) tab.push_back(make_float2((float)t.x, (float)t.y));
             if ((rc = upload((void **)&pl->d_fft_tables, tab.data(), sizeof(float2) * tab.size())))
+                return rc;
+            // exact ranges requested: the forward's transforms in FP64 as well
+            if (pl->requested_precision == SARHIP_PRECISION_FP64 &&
+                (rc = upload((void **)&pl->d_fft_tables64, tab64.data(),
+                             sizeof(double2) * tab64.size())))
                 return rc;
             pl->fused_U = (int)U;
             // rows of invalid pulses are never read: empty spans
@@ -1238,7 +1254,7 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
         if ((rc = record(pl, 4, s))) return rc;
         HIP_TRY(launch_fp_bin(k, s));
         if ((rc = record(pl, 5, s))) return rc;
-        if (pl->win > 0) HIP_TRY(launch_fp_reduce(k, pl->reduce_chunk, s));
+        if (pl->win > 0) HIP_TRY(launch_fp_reduce(k, s));
     } else {
         if ((rc = record(pl, 4, s))) return rc;
         if ((rc = record(pl, 5, s))) return rc;
@@ -1248,7 +1264,7 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
         // FFT and truncation in one pass over the profile rows
         HIP_TRY(launch_fft_down(pl->d_prof, spans ? pl->d_fspan : nullptr, pl->d_W, pl->d_wfac_fwd,
                                 (int)pl->N, pl->fused_U, (int)pl->P, order == SARHIP_ORDER_CENTRED,
-                                pl->d_fft_tables, (float2 *)phase_history,
+                                pl->d_fft_tables, pl->d_fft_tables64, (float2 *)phase_history,
                                 spans && pl->fft_sparse, s));
         if ((rc = record(pl, 2, s))) return rc;
         if ((rc = record(pl, 3, s))) return rc;
