@@ -14,6 +14,8 @@
 // truncate, which is exact by linearity of the FFT.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "sar_device.h"
 #include "sar_launch.h"
 
@@ -826,6 +828,7 @@ __global__ __launch_bounds__(kThreads) void region_l1_kernel(float2 *__restrict_
     }
 }
 
+constexpr int kScatGroup = 4;  // pixel records per scalar-load group (compact path)
 template <int INTERP, bool MONO, int PREC, bool PLANAR, int NT, int SORD = 5>
 __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     // Fixed-point windows [G][wst] of bins, a bin being two 32-bit words
@@ -864,10 +867,12 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const double rho = k.kc_rev / k.by_dx;  // revolutions per bin
     // region bound (region_l1_kernel) -> fixed-point scale
     const float bound = k.region_l1[region];
-    const bool fixed_ok = fix_bound_ok(bound) && win > 0;
-    const bool zero_region = fixed_ok && bound == 0.f;
+    // region-uniform flags, made scalar so that they branch on the scalar unit
+    const bool fixed_ok =
+        __builtin_amdgcn_readfirstlane((int)(fix_bound_ok(bound) && win > 0)) != 0;
+    const bool zero_region = fixed_ok && __builtin_amdgcn_readfirstlane((int)(bound == 0.f)) != 0;
     // amplitudes arrive pre-scaled by `fs` when `scaled`
-    const bool scaled = fixed_ok && bound > 0.f;
+    const bool scaled = fixed_ok && !zero_region;
     FixScale fs{1.f, 1.f, 0};
     if (scaled) fs = fix_scale(bound);
     auto unscale = [&](const float v) { return scaled ? fix_undo(fs, v) : v; };
@@ -880,76 +885,100 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     __syncthreads();
     unsigned int *ws = (unsigned int *)(wsum + q * wst);
     float2 *row = k.prof + (size_t)p * k.nw;
-    if (!zero_region) {  // skip all-zero regions
-        for (int tile = t0; tile < t1; ++tile) {
-            PulseData<MONO> d;
-            load_pulse<PREC, MONO>(k, tile, j, p, d);
-            if constexpr (kRel) d.s.cu -= wb;
-            SeriesHorner<MONO, SORD> hc;
-            if constexpr (PREC == kPrecSeries) hc = make_horner<MONO, SORD>(d, rho);
-            const int tb = tile * kTilePixels;
-            const int tn = min(kTilePixels, k.M - tb);
-            // NT == 512: the next pixel's record is requested one iteration ahead.
-            // kCompact: each wave walks a contiguous block of the tile's
-            // pixels through the 32-byte records (one scalar load per pixel,
-            // two records per cache line); otherwise the streams interleave.
-            PixRec r_next;
-            float2 a_next;
-            const int per = (tn + nstreams - 1) / nstreams;
-            const int ibeg = kCompact ? stream * per : stream;
-            const int iend = kCompact ? min(tn, ibeg + per) : tn;
-            const int istep = kCompact ? 1 : nstreams;
-            if constexpr (kCompact) {
-                // (loaded in pairs below)
-            } else if constexpr (NT == 512) {
-                if (stream < tn) {
-                    r_next = load_uniform(k.pixrec, (size_t)(tb + stream));
-                    a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + stream));
+    bool oob = false;  // a contribution of this lane fell outside its window
+    // The pixel loops, instantiated for regions with windows (kFixed) and
+    // without, so that the per-pixel code of each has no region test.
+    auto scan = [&](auto fixed_tag) {
+        constexpr bool kFixed = decltype(fixed_tag)::value;
+            for (int tile = t0; tile < t1; ++tile) {
+                PulseData<MONO> d;
+                load_pulse<PREC, MONO>(k, tile, j, p, d);
+                if constexpr (kRel) d.s.cu -= wb;
+                SeriesHorner<MONO, SORD> hc;
+                if constexpr (PREC == kPrecSeries) hc = make_horner<MONO, SORD>(d, rho);
+                const int tb = tile * kTilePixels;
+                const int tn = min(kTilePixels, k.M - tb);
+                // NT == 512: the next pixel's record is requested one iteration ahead.
+                // kCompact: each wave walks a contiguous block of the tile's
+                // pixels through the 32-byte records (one scalar load per pixel,
+                // two records per cache line); otherwise the streams interleave.
+                PixRec r_next;
+                float2 a_next;
+                // kCompact: blocks of whole record groups, so that no group
+                // straddles two streams (only the image's last group may end
+                // past its last pixel, in the record array's spare records)
+                const int per = kCompact ? (tn + nstreams * kScatGroup - 1) / (nstreams * kScatGroup) * kScatGroup
+                                         : (tn + nstreams - 1) / nstreams;
+                const int ibeg = kCompact ? stream * per : stream;
+                const int iend = kCompact ? min(tn, ibeg + per) : tn;
+                const int istep = kCompact ? 1 : nstreams;
+                if constexpr (kCompact) {
+                    // (loaded in pairs below)
+                } else if constexpr (NT == 512) {
+                    if (stream < tn) {
+                        r_next = load_uniform(k.pixrec, (size_t)(tb + stream));
+                        a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + stream));
+                    }
                 }
-            }
-            // one pixel's contribution to this lane's pulse window
-            auto contribute = [&](const Pix &px, const float2 a) {
-                double u, th;
-                if constexpr (PREC == kPrecSeries)
-                    eval_range_horner<MONO, PLANAR, SORD>(d, hc, px, rho, u, th);
-                else
-                    eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
-                float sn, cs;
-                double fl;
-                if constexpr (INTERP == kInterpLinear && kRel) {
-                    // u is taken from the window base, at least 2 bins below
-                    // every pixel of the region: truncation is floor
-                    fl = (double)(int)u;
-                } else if constexpr (INTERP == kInterpLinear) {
-                    fl = floor(u);
-                } else {
-                    // forward bin rule round(u) for both nearest modes (CPP:289)
-                    fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
-                }
-                // exact ranges: the phase argument's FP32 rounding is corrected
-                if constexpr (PREC == kPrecFp64)
-                    phasor_fract_fine(th, &sn, &cs);
-                else
-                    phasor_fract(th, &sn, &cs);
-                // contribution a e^{i theta} (times `scale` when pre-scaled)
-                const float cr = fmaf(cs, a.x, -(sn * a.y));
-                const float ci = fmaf(cs, a.y, sn * a.x);
-                const int l = kRel ? (int)fl : (int)(fl - wb);
-                if constexpr (INTERP == kInterpLinear) {
-                    const float fr = kRel ? (float)__builtin_amdgcn_fract(u) : (float)(u - fl);
-                    if (fixed_ok && (unsigned)l < (unsigned)(win - 1)) {
-                        if (qok) {
+                // one pixel's contribution to this lane's pulse window
+                auto contribute = [&](const Pix &px, const float2 a) {
+                    double u, th;
+                    if constexpr (PREC == kPrecSeries)
+                        eval_range_horner<MONO, PLANAR, SORD>(d, hc, px, rho, u, th);
+                    else
+                        eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
+                    float sn, cs;
+                    double fl;
+                    if constexpr (INTERP == kInterpLinear && kRel) {
+                        // u is taken from the window base, at least 2 bins below
+                        // every pixel of the region: truncation is floor
+                        fl = (double)(int)u;
+                    } else if constexpr (INTERP == kInterpLinear) {
+                        fl = floor(u);
+                    } else {
+                        // forward bin rule round(u) for both nearest modes (CPP:289)
+                        fl = nearest_bin<kInterpNearest, PREC, MONO>(k, px, p, u, th);
+                    }
+                    // exact ranges: the phase argument's FP32 rounding is corrected
+                    if constexpr (PREC == kPrecFp64)
+                        phasor_fract_fine(th, &sn, &cs);
+                    else
+                        phasor_fract(th, &sn, &cs);
+                    // contribution a e^{i theta} (times `scale` when pre-scaled)
+                    const float cr = fmaf(cs, a.x, -(sn * a.y));
+                    const float ci = fmaf(cs, a.y, sn * a.x);
+                    const int l = kRel ? (int)fl : (int)(fl - wb);
+                    if constexpr (kFixed) {
+                        // Every lane adds into its window without a branch: a lane
+                        // whose bin falls outside the window adds at the spare
+                        // bins at its end (for linear at l = win - 1 into the last
+                        // bin and a spare one; the row is poisoned then anyway)
+                        // and is flagged (oob, handled after the pixel loops);
+                        // lanes without a pulse (q >= np) add into windows never
+                        // written out.
+                        if constexpr (INTERP == kInterpLinear) {
+                            const float fr = kRel ? (float)__builtin_amdgcn_fract(u) : (float)(u - fl);
+                            const bool in = (unsigned)l < (unsigned)(win - 1);
+                            const int lt = (int)min((unsigned)l, (unsigned)win);
                             const int tr = fix_tz(cr), ti = fix_tz(ci);
                             const int r1 = fix_tz(fr * cr), i1 = fix_tz(fr * ci);
-                            add_fix(ws + 2 * l, tr - r1, ti - i1);
-                            add_fix(ws + 2 * l + 2, r1, i1);
+                            add_fix(ws + 2 * lt, tr - r1, ti - i1);
+                            add_fix(ws + 2 * lt + 2, r1, i1);
+                            oob = oob || !in;
+                        } else {
+                            const bool in = (unsigned)l < (unsigned)win;
+                            add_fix(ws + 2 * (int)min((unsigned)l, (unsigned)win), fix_tz(cr), fix_tz(ci));
+                            oob = oob || !in;
                         }
                     } else if (qok) {
+                        // a region without windows (non-finite amplitudes, or no
+                        // window fits the LDS): float global atomics into the
+                        // cleared rows, or on the span path (rows not cleared) a
+                        // poisoned row (NaN output row)
                         if (k.fwd_spans) {
-                            // rows are not cleared and the merge stores spans:
-                            // the row is poisoned instead (NaN output row)
                             atomicOr(&k.fpoison[p], 1);
-                        } else {
+                        } else if constexpr (INTERP == kInterpLinear) {
+                            const float fr = kRel ? (float)__builtin_amdgcn_fract(u) : (float)(u - fl);
                             const float gr = 1.0f - fr;
                             const int n0 = bin_index(k, kRel ? floor(u) + wb : fl, 1);
                             const int n1 = n0 + 1 == k.nw ? 0 : n0 + 1;
@@ -957,89 +986,94 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             atomicAdd(&row[n0].y, unscale(gr * ci));
                             atomicAdd(&row[n1].x, unscale(fr * cr));
                             atomicAdd(&row[n1].y, unscale(fr * ci));
-                        }
-                    }
-                } else {
-                    if (fixed_ok && (unsigned)l < (unsigned)win) {
-                        if (qok) add_fix(ws + 2 * l, fix_tz(cr), fix_tz(ci));
-                    } else if (qok) {
-                        if (k.fwd_spans) {
-                            atomicOr(&k.fpoison[p], 1);
                         } else {
                             const int nb = bin_index(k, fl, 1);
                             atomicAdd(&row[nb].x, unscale(cr));
                             atomicAdd(&row[nb].y, unscale(ci));
                         }
                     }
-                }
-            };
-            if constexpr (kCompact) {
-                // records in groups of four (two 64-byte scalar loads), the
-                // next group requested while this one is processed: one wait
-                // per four pixels; invalid pixels carry a zero amplitude and
-                // a zero offset (the record array has four spare records at
-                // its end, so a group never reads past it)
-                constexpr int KB = 4;
-                ScatRec c[KB];
-                if (ibeg < iend) {
-                    load_srec2(k.srec, (size_t)(tb + ibeg), c[0], c[1]);
-                    load_srec2(k.srec, (size_t)(tb + ibeg + 2), c[2], c[3]);
-                }
-                for (int i = ibeg; i < iend; i += KB) {
-                    ScatRec cur[KB];
-#pragma unroll
-                    for (int e = 0; e < KB; ++e) cur[e] = c[e];
-                    if (i + KB < iend) {
-                        load_srec2(k.srec, (size_t)(tb + i + KB), c[0], c[1]);
-                        load_srec2(k.srec, (size_t)(tb + i + KB + 2), c[2], c[3]);
-                    }
-                    Pix px;
-                    px.x = px.y = px.z = 0.0;  // unused on this path
-                    px.dz = 0.0;
-                    px.ok = true;
-#pragma unroll
-                    for (int e = 0; e < KB; ++e) {
-                        if (i + e < iend) {
-                            px.dx = cur[e].dx;
-                            px.dy = cur[e].dy;
-                            px.d2 = cur[e].d2;
-                            contribute(px, cur[e].a);
+                };
+                if constexpr (kCompact) {
+                    // records in groups of four (two 64-byte scalar loads), the
+                    // next group requested while this one is processed: one wait
+                    // per four pixels; invalid pixels carry a zero amplitude and
+                    // a zero offset (the record array has four spare records at
+                    // its end, so a group never reads past it, and a spare
+                    // record adds zero).  Two group buffers alternate, so no
+                    // registers are copied between groups.
+                    constexpr int KB = kScatGroup;
+                    ScatRec ca[KB], cb[KB];
+                    auto load_group = [&](ScatRec(&c)[KB], const int i) {
+                        load_srec2(k.srec, (size_t)(tb + i), c[0], c[1]);
+                        load_srec2(k.srec, (size_t)(tb + i + 2), c[2], c[3]);
+                    };
+                    auto run_group = [&](const ScatRec(&c)[KB], const int i) {
+                        Pix px;
+                        px.x = px.y = px.z = 0.0;  // unused on this path
+                        px.dz = 0.0;
+                        px.ok = true;
+    #pragma unroll
+                        for (int e = 0; e < KB; ++e) {
+                            px.dx = c[e].dx;
+                            px.dy = c[e].dy;
+                            px.d2 = c[e].d2;
+                            contribute(px, c[e].a);
                         }
+                    };
+                    if (ibeg < iend) load_group(ca, ibeg);
+                    for (int i = ibeg; i < iend; i += 2 * KB) {
+                        if (i + KB < iend) load_group(cb, i + KB);
+                        run_group(ca, i);
+                        if (i + KB >= iend) break;
+                        if (i + 2 * KB < iend) load_group(ca, i + 2 * KB);
+                        run_group(cb, i + KB);
                     }
-                }
-            } else {
-                for (int i = ibeg; i < iend; i += istep) {
-                    PixRec r;
-                    float2 a;
-                    if constexpr (NT == 512) {
-                        // wave-uniform index: scalar loads through the constant address space
-                        r = r_next;
-                        a = a_next;
-                        if (i + nstreams < tn) {
-                            r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
-                            a_next = load_uniform((const float2 *)k.amps_perm,
-                                                  (size_t)(tb + i + nstreams));
+                } else {
+                    for (int i = ibeg; i < iend; i += istep) {
+                        PixRec r;
+                        float2 a;
+                        if constexpr (NT == 512) {
+                            // wave-uniform index: scalar loads through the constant address space
+                            r = r_next;
+                            a = a_next;
+                            if (i + nstreams < tn) {
+                                r_next = load_uniform(k.pixrec, (size_t)(tb + i + nstreams));
+                                a_next = load_uniform((const float2 *)k.amps_perm,
+                                                      (size_t)(tb + i + nstreams));
+                            }
+                            // ok is exactly 0.0 or 1.0: an integer test keeps it on the scalar unit
+                            if (__double_as_longlong(r.ok) == 0) continue;
+                        } else {
+                            r = k.pixrec[tb + i];
+                            if (r.ok == 0.0) continue;
+                            a = k.amps_perm[tb + i];
                         }
-                        // ok is exactly 0.0 or 1.0: an integer test keeps it on the scalar unit
-                        if (__double_as_longlong(r.ok) == 0) continue;
-                    } else {
-                        r = k.pixrec[tb + i];
-                        if (r.ok == 0.0) continue;
-                        a = k.amps_perm[tb + i];
+                        Pix px;
+                        px.x = r.x;
+                        px.y = r.y;
+                        px.z = r.z;
+                        px.dx = r.dx;
+                        px.dy = r.dy;
+                        px.dz = r.dz;
+                        px.d2 = r.d2;
+                        px.ok = true;
+                        contribute(px, a);
                     }
-                    Pix px;
-                    px.x = r.x;
-                    px.y = r.y;
-                    px.z = r.z;
-                    px.dx = r.dx;
-                    px.dy = r.dy;
-                    px.dz = r.dz;
-                    px.d2 = r.d2;
-                    px.ok = true;
-                    contribute(px, a);
                 }
             }
-        }
+    };
+    if (fixed_ok && !zero_region)
+        scan(std::true_type{});
+    else if (!fixed_ok)
+        scan(std::false_type{});  // zero regions are skipped (exact)
+    // A bin outside the window is ruled out by the window bounds; should one
+    // occur, the row is made NaN (poisoned on the span path, a NaN added to the
+    // cleared row otherwise) rather than left silently wrong.
+    if (oob && qok) {
+        if (k.fwd_spans)
+            atomicOr(&k.fpoison[p], 1);
+        else
+            atomicAdd(&row[0].x, __builtin_nanf(""));
     }
     __syncthreads();
     // Partials and per-window maxima: wave w writes the integer windows of

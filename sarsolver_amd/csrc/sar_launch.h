@@ -8,6 +8,11 @@
 namespace sarhip {
 
 // Everything a gather / scatter kernel reads, passed by value.
+// Per-pulse stride (in bins) of the forward's LDS windows: the window's bins,
+// two spare bins that out-of-window contributions are sent to (never read),
+// rounded up to an odd count (consecutive pulses start in different banks).
+inline int fwd_window_stride(int win) { return (win + 2) | 1; }
+
 struct KernelParams {
     // pixels, in tile (Morton) order
     const double *px, *py, *pz;  // FP64 SoA positions
@@ -41,7 +46,7 @@ struct KernelParams {
     int fwd_pulses;           // pulses (LDS windows) per forward workgroup, power of 2
     int fwd_groups;           // ceil(Pv / fwd_pulses)
     int win;                  // LDS window length per pulse (0 = no windows)
-    int win_stride;           // odd per-pulse stride of the LDS window planes (>= win)
+    int win_stride;           // odd per-pulse stride of the LDS windows (fwd_window_stride)
     int2 *partial;            // [Pv][num_regions][win] integer window sums
     double *wbases;           // [Pv][num_regions]
     float *wmax;              // [Pv][num_regions] max |component| of each window (its units)

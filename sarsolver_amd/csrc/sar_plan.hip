@@ -234,7 +234,7 @@ struct sarhip_plan_s {
         k.fwd_pulses = fwd_pulses;
         k.fwd_groups = fwd_groups;
         k.win = win;
-        k.win_stride = win | 1;
+        k.win_stride = fwd_window_stride(win);
         k.partial = d_partial;
         k.wbases = d_wbases;
         k.wmax = d_wmax;
@@ -836,7 +836,7 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                     if (forced_g > 0 && G != forced_g) continue;
                     if ((pass == 0) != (G == 64)) continue;
                     const size_t lim = G == 64 ? budget64 : budget;
-                    if ((size_t)G * (w | 1) * sizeof(unsigned long long) > lim) continue;
+                    if ((size_t)G * fwd_window_stride(w) * sizeof(unsigned long long) > lim) continue;
                     const double cost = (double)w / R * (G >= 32 ? 1.0 : 4.0);
                     if (cost < best_cost) {
                         best_cost = cost;
@@ -914,9 +914,13 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     }
     // The planar 64-pulse linear scatter streams 32-byte records that the
     // region-norm kernel writes on every forward call.
+    // The image's last record group may run into four spare records, which
+    // are zero (zero offset and amplitude: a zero contribution).
     if (pl->planar && interp == SARHIP_INTERP_LINEAR && pl->fwd_pulses == 64 &&
-        (pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO))
-        HIP_TRY(hipMalloc((void **)&pl->d_srec, sizeof(ScatRec) * (M + 4)));  // + group overrun
+        (pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO)) {
+        HIP_TRY(hipMalloc((void **)&pl->d_srec, sizeof(ScatRec) * (M + 4)));
+        HIP_TRY(hipMemset(pl->d_srec + M, 0, sizeof(ScatRec) * 4));
+    }
     // Backprojection: split each tile's pulses over several workgroups until
     // the grid has about 32 workgroups per CU, keeping at least 128 pulses per
     // split; partial images are summed in FP64 in split order.  (C2, 1024
