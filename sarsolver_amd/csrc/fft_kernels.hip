@@ -381,13 +381,16 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
 }
 
 // Thread groups of the forward: one transform per group of T threads, one or
-// two groups per workgroup as for Batch.  T = 256, and in FP64 max(256, N / 16)
-// (at most one radix-16 butterfly per thread and pass, which keeps the FP64
-// registers of N = 8192 in bounds).
+// two groups per workgroup as for Batch.  T = 512 at N = 8192 (one radix-16
+// butterfly per thread and pass) and for FP64 at N = 4096 (twice the waves on
+// the one 68 KiB buffer a CU can hold two of), else 256.  Measured at
+// 2048 / 4096 rows (forward spectral stage): FP32 N = 8192 1.49 -> 0.99 ms,
+// FP64 N = 4096 0.49 -> 0.33 ms; 512 threads made FP32 N = 4096 (0.20 ->
+// 0.31 ms) and N = 2048 slower.
 template <int N, int Q, typename C>
 struct DownBatch {
     static constexpr bool kF64 = std::is_same<C, double2>::value;
-    static constexpr int T = kF64 && N / 16 > 256 ? N / 16 : 256;
+    static constexpr int T = N >= 8192 || (kF64 && N >= 4096) ? 512 : 256;
     static constexpr int G = Q >= 2 ? 2 : 1;
     static constexpr int PPG = Q / G;
     static constexpr int NT = T * G;
