@@ -281,17 +281,28 @@ __device__ __forceinline__ void load_phases(const float2 *src, float2 (&v)[Q]) {
 template <int N, int Q>
 constexpr int kUpWaves = (N <= 4096 && Q <= 2) ? 3 : 1;
 
+// Thread groups of the adjoint: as Batch, with T = 512 threads per transform
+// at N = 8192 with one phase per batch (one radix-16 butterfly per thread and
+// pass; two groups of 512 would leave 128 registers per thread), else 256.
 template <int N, int Q>
-__global__ __launch_bounds__(Batch<Q>::NT) __attribute__((amdgpu_waves_per_eu(kUpWaves<N, Q>)))
+struct UpBatch {
+    static constexpr int T = N >= 8192 && Q == 1 ? 512 : 256;
+    static constexpr int G = Q >= 2 ? 2 : 1;
+    static constexpr int PPG = Q / G;
+    static constexpr int NT = T * G;
+};
+template <int N, int Q>
+__global__ __launch_bounds__((UpBatch<N, Q>::NT)) __attribute__((amdgpu_waves_per_eu(kUpWaves<N, Q>)))
 void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
                    const float2 *__restrict__ Wconj, const float2 *__restrict__ wfac, int U, int ld,
                    int centred, const float2 *__restrict__ tables, float2 *__restrict__ prof) {
-    using Bt = Batch<Q>;
+    using Bt = UpBatch<N, Q>;
+    constexpr int T = Bt::T;
     extern __shared__ float2 lds[];
     constexpr int SZ = N + N / 16;
-    constexpr int NI = (N + 255) / 256;
+    constexpr int NI = (N + T - 1) / T;
     const int p = blockIdx.x;
-    const int tid = threadIdx.x, g = tid >> 8, gt = tid & 255;
+    const int tid = threadIdx.x, g = tid / T, gt = tid % T;
     const int nw = U * N;
     const Twiddles<float2> tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
     const float2 wp = wfac[p];
@@ -302,7 +313,7 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
     float2 B[NI];
 #pragma unroll
     for (int i = 0; i < NI; ++i) {
-        const int k = gt + 256 * i;
+        const int k = gt + T * i;
         if (k < N) {
             const int src = centred ? (k + N / 2) % N : k;
             B[i] = cmulf2(cmulf2(Wconj[k], y[src]), wp);
@@ -314,31 +325,31 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
         for (int pp = 0; pp < Bt::PPG; ++pp) {
             const int buf = g + Bt::G * pp, rp = r0 + buf;
             float2 *sh = lds + buf * SZ;
-            if constexpr (N >= 4096) {
+            if constexpr (N >= 16 * T) {
                 // the first radix-16 pass straight from registers: butterfly
-                // j = gt + 256 b takes k = j + (N / 16) r, held in B[b + (N / 4096) r]
-                constexpr int PB = N / 4096;
+                // j = gt + T b takes k = j + (N / 16) r, held in B[b + PB r]
+                constexpr int PB = N / (16 * T);
 #pragma unroll
                 for (int b = 0; b < PB; ++b) {
                     float2 v[16];
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int k = gt + 256 * (b + PB * r);
+                        const int k = gt + T * (b + PB * r);
                         int t = (signed_freq(k, N) * rp) % nw;
                         t = t < 0 ? t + nw : t;
                         v[r] = cmulf2(B[b + PB * r], conjf2(tw.w(t)));
                     }
                     dft<16, true>(v);
-                    const int j = gt + 256 * b;
+                    const int j = gt + T * b;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) sh[phys(16 * j + r)] = v[r];
                 }
                 __syncthreads();
-                fft_passes<N, 16, true, 256>(sh, tw, gt);
+                fft_passes<N, 16, true, T>(sh, tw, gt);
             } else {
 #pragma unroll
                 for (int i = 0; i < NI; ++i) {
-                    const int k = gt + 256 * i;
+                    const int k = gt + T * i;
                     if (k < N) {
                         int t = (signed_freq(k, N) * rp) % nw;
                         t = t < 0 ? t + nw : t;
@@ -346,7 +357,7 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
                     }
                 }
                 __syncthreads();
-                fft_lds<N, true>(sh, tw, gt);
+                fft_lds<N, true, T>(sh, tw, gt);
             }
         }
         // Q consecutive phases of each m are contiguous in the row
@@ -381,16 +392,17 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
 }
 
 // Thread groups of the forward: one transform per group of T threads, one or
-// two groups per workgroup as for Batch.  T = 512 at N = 8192 (one radix-16
-// butterfly per thread and pass) and for FP64 at N = 4096 (twice the waves on
-// the one 68 KiB buffer a CU can hold two of), else 256.  Measured at
+// two groups per workgroup as for Batch.  With one phase per batch, T = 512 at
+// N = 8192 (one radix-16 butterfly per thread and pass) and for FP64 at
+// N = 4096 (twice the waves on the one 68 KiB buffer a CU can hold two of),
+// else 256.  Measured at
 // 2048 / 4096 rows (forward spectral stage): FP32 N = 8192 1.49 -> 0.99 ms,
 // FP64 N = 4096 0.49 -> 0.33 ms; 512 threads made FP32 N = 4096 (0.20 ->
 // 0.31 ms) and N = 2048 slower.
 template <int N, int Q, typename C>
 struct DownBatch {
     static constexpr bool kF64 = std::is_same<C, double2>::value;
-    static constexpr int T = N >= 8192 || (kF64 && N >= 4096) ? 512 : 256;
+    static constexpr int T = Q == 1 && (N >= 8192 || (kF64 && N >= 4096)) ? 512 : 256;
     static constexpr int G = Q >= 2 ? 2 : 1;
     static constexpr int PPG = Q / G;
     static constexpr int NT = T * G;
@@ -694,8 +706,8 @@ hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const flo
     hipError_t e = hipFuncSetAttribute((const void *)fft_up_kernel<N, Q>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    fft_up_kernel<N, Q><<<P, Batch<Q>::NT, lds, s>>>(spans, Y, Wconj, wfac, U, ld, centred, tables,
-                                                      prof);
+    fft_up_kernel<N, Q><<<P, UpBatch<N, Q>::NT, lds, s>>>(spans, Y, Wconj, wfac, U, ld, centred,
+                                                           tables, prof);
     return hipGetLastError();
 }
 
