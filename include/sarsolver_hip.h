@@ -61,7 +61,11 @@ typedef struct SarCalculationInfo {
 
 /* Range / phase arithmetic. */
 #define SARHIP_PRECISION_FP64 0   /* platform-to-pixel ranges in FP64 exactly as
-                                     the reference evaluates them             */
+                                     the reference evaluates them; with fused
+                                     spectral stages the forward's runs in
+                                     FP64 and its phasors carry the phase's
+                                     FP32 rounding (for scenes whose phase
+                                     history cancels heavily)                 */
 #define SARHIP_PRECISION_SERIES 1 /* FP64 tile-centred series for the range;
                                      exact FP64 re-evaluation at bin ties     */
 #define SARHIP_PRECISION_HILO 2   /* FP64 tile-centre range + FP32 differential
