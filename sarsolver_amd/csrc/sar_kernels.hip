@@ -116,6 +116,33 @@ __global__ __launch_bounds__(256) void truncate_kernel(const float2 *__restrict_
     Y[(size_t)p * N + dst] = v;
 }
 
+// The same from FP64 rows (the exact-range mode's rocFFT forward), products
+// in FP64 and one rounding of each output.
+__global__ __launch_bounds__(256) void truncate64_kernel(const double2 *__restrict__ G,
+                                                         const float2 *__restrict__ W,
+                                                         const float2 *__restrict__ wfac, int N,
+                                                         int nw, int centred,
+                                                         float2 *__restrict__ Y) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const int p = blockIdx.y;
+    if (k >= N) return;
+    const int lo = (N + 1) / 2;
+    const int src = k < lo ? k : nw - N + k;
+    const double2 g = G[(size_t)p * nw + src];
+    const double wr = W[k].x, wi = W[k].y, pr = wfac[p].x, pi = wfac[p].y;
+    const double ar = fma(wr, g.x, -wi * g.y), ai = fma(wr, g.y, wi * g.x);
+    const int dst = centred ? (k + N / 2) % N : k;
+    Y[(size_t)p * N + dst] = make_float2((float)fma(ar, pr, -ai * pi), (float)fma(ar, pi, ai * pr));
+}
+
+// dst[i] = src[i] in FP64 (profile rows into the FP64 transform's buffer).
+__global__ __launch_bounds__(256) void widen_kernel(const float2 *__restrict__ src,
+                                                    double2 *__restrict__ dst, size_t n) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride)
+        dst[i] = make_double2((double)src[i].x, (double)src[i].y);
+}
+
 // Zero `bytes` bytes at a 16-byte aligned `buf`: 16-byte stores, then the
 // remaining bytes % 16 (e.g. the last complex64 of an odd element count).
 __global__ __launch_bounds__(256) void fill_zero_kernel(float4 *__restrict__ buf, size_t bytes) {
@@ -1329,6 +1356,20 @@ hipError_t launch_truncate(const float2 *G, const float2 *W, const float2 *wfac,
                            int P, int centred, float2 *Y, hipStream_t s) {
     dim3 grid((N + 255) / 256, P);
     truncate_kernel<<<grid, 256, 0, s>>>(G, W, wfac, N, nw, centred, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_truncate64(const double2 *G, const float2 *W, const float2 *wfac, int N, int nw,
+                             int P, int centred, float2 *Y, hipStream_t s) {
+    dim3 grid((N + 255) / 256, P);
+    truncate64_kernel<<<grid, 256, 0, s>>>(G, W, wfac, N, nw, centred, Y);
+    return hipGetLastError();
+}
+
+hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s) {
+    const size_t blocks = std::min<size_t>((n + 255) / 256, 8192);
+    if (blocks == 0) return hipSuccess;
+    widen_kernel<<<(unsigned)blocks, 256, 0, s>>>(src, dst, n);
     return hipGetLastError();
 }
 

@@ -103,6 +103,10 @@ hipError_t launch_window_pad(const float2 *Y, const float2 *Wconj, const float2 
 hipError_t launch_wrap_pad(float2 *prof, int nw, int ld, int P, hipStream_t s);
 hipError_t launch_truncate(const float2 *G, const float2 *W, const float2 *wfac, int N, int nw,
                            int P, int centred, float2 *Y, hipStream_t s);
+// the same from FP64 rows, and FP64 copies of complex64 rows
+hipError_t launch_truncate64(const double2 *G, const float2 *W, const float2 *wfac, int N, int nw,
+                             int P, int centred, float2 *Y, hipStream_t s);
+hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s);
 hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 *dst,
                               hipStream_t s);

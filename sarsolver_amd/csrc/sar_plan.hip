@@ -147,6 +147,8 @@ struct sarhip_plan_s {
     int bp_splits = 1;
 
     rocfft_plan fft_fwd = nullptr, fft_bwd = nullptr;
+    rocfft_plan fft_fwd64 = nullptr;  // exact ranges requested: the forward in FP64
+    double2 *d_prof64 = nullptr;      // [P][N_w] its rows
     rocfft_execution_info fft_info = nullptr;
     void *fft_work = nullptr;
     size_t fft_work_size = 0;
@@ -193,6 +195,8 @@ struct sarhip_plan_s {
         if (fft_work) (void)hipFree(fft_work);
         if (fft_fwd) rocfft_plan_destroy(fft_fwd);
         if (fft_bwd) rocfft_plan_destroy(fft_bwd);
+        if (fft_fwd64) rocfft_plan_destroy(fft_fwd64);
+        dev_free(d_prof64);
         if (fft_info) rocfft_execution_info_destroy(fft_info);
         for (auto &pool : ev_pool)
             for (auto &e : pool)
@@ -1028,10 +1032,20 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         rocfft_plan_description_destroy(desc);
         FFT_TRY(st);
     }
+    // exact ranges requested: the forward's transform in FP64 as in the fused
+    // kernels (the rows are widened before it)
+    size_t w3 = 0;
+    if (pl->requested_precision == SARHIP_PRECISION_FP64) {
+        FFT_TRY(rocfft_plan_create(&pl->fft_fwd64, rocfft_placement_inplace,
+                                   rocfft_transform_type_complex_forward, rocfft_precision_double,
+                                   1, &len, P, nullptr));
+        FFT_TRY(rocfft_plan_get_work_buffer_size(pl->fft_fwd64, &w3));
+        HIP_TRY(hipMalloc((void **)&pl->d_prof64, sizeof(double2) * P * nw));
+    }
     size_t w1 = 0, w2 = 0;
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl->fft_fwd, &w1));
     FFT_TRY(rocfft_plan_get_work_buffer_size(pl->fft_bwd, &w2));
-    pl->fft_work_size = std::max(w1, w2);
+    pl->fft_work_size = std::max(std::max(w1, w2), w3);
     FFT_TRY(rocfft_execution_info_create(&pl->fft_info));
     if (pl->fft_work_size) {
         HIP_TRY(hipMalloc(&pl->fft_work, pl->fft_work_size));
@@ -1271,6 +1285,16 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
                                 pl->d_fft_tables, pl->d_fft_tables64, (float2 *)phase_history,
                                 spans && pl->fft_sparse, s));
         if ((rc = record(pl, 2, s))) return rc;
+        if ((rc = record(pl, 3, s))) return rc;
+        return SARHIP_OK;
+    }
+    if (pl->fft_fwd64) {
+        HIP_TRY(launch_widen(pl->d_prof, pl->d_prof64, pl->P * pl->nw, s));
+        if ((rc = run_fft(pl, pl->fft_fwd64, pl->d_prof64, s))) return rc;
+        if ((rc = record(pl, 2, s))) return rc;
+        HIP_TRY(launch_truncate64(pl->d_prof64, pl->d_W, pl->d_wfac_fwd, (int)pl->N, (int)pl->nw,
+                                  (int)pl->P, order == SARHIP_ORDER_CENTRED,
+                                  (float2 *)phase_history, s));
         if ((rc = record(pl, 3, s))) return rc;
         return SARHIP_OK;
     }

@@ -8,10 +8,10 @@ is a sum that cancels heavily (the band keeps ~1/20 of the profile row's norm
 in the worst looks), so FP32 rounding anywhere before the band is formed --
 phasors, products, the fixed-point windows, the spectral stage -- is amplified
 as much.  precision="fp64" (exact ranges) carries the phase argument's FP32
-rounding into the phasor and runs the forward's spectral stage in FP64, and
-holds every scene to 1e-5.  The default precision="series" keeps both in FP32
-for speed; its worst look measured 1.12e-5 (the next 9.9e-6), held here to
-1.5e-5.  Both modes merge the windows exactly (64-bit integers), and both are
+rounding into the phasor and runs the forward's spectral stage in FP64 (the
+fused kernel or rocFFT: 7.4e-6 worst look either way), and holds every scene
+to 1e-5.  The default precision="series" keeps both in FP32 for speed; its
+worst look measured 1.12e-5 (the next 9.9e-6), held here to 1.5e-5.  Both modes merge the windows exactly (64-bit integers), and both are
 held to a normwise error ||dY|| / (||A|| ||x||) <= 1e-6, ||A||^2 = P N for unit
 weights.
 """
@@ -59,17 +59,24 @@ def scenes(oracle_lib):
     return w, pos, rows, out
 
 
-@pytest.mark.parametrize("precision", ["fp64", "series"])
-def test_c4_forward_every_scene(dev, scenes, precision):
+@pytest.mark.parametrize("precision,fused", [("fp64", True), ("series", True), ("fp64", False)])
+def test_c4_forward_every_scene(dev, scenes, precision, fused):
+    """fused=False: rocFFT spectral stages (FP64 for the forward in fp64 mode)."""
+    import os
     from sarsolver_amd import SarPlan
     w, pos, rows, data = scenes
     errs = []
     for b, (start, ap, x, want) in enumerate(data):
-        print(f"case: c4 forward scene {b} precision {precision}", flush=True)
-        pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
-                     np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting,
-                     ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, w.upsample, 1.0,
-                     interp="linear", precision=precision, device=dev)
+        print(f"case: c4 forward scene {b} precision {precision} fused {fused}", flush=True)
+        os.environ["SARHIP_FUSED_FFT"] = "1" if fused else "0"
+        try:
+            pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
+                         np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting,
+                         ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, w.upsample, 1.0,
+                         interp="linear", precision=precision, device=dev)
+        finally:
+            os.environ.pop("SARHIP_FUSED_FFT", None)
+        assert pl.layout["fused_fft_ratio"] == (w.upsample if fused else 0)
         with torch.no_grad():
             got = pl.forward(torch.from_numpy(x).to(dev)).cpu().numpy()[rows]
         del pl
@@ -78,5 +85,5 @@ def test_c4_forward_every_scene(dev, scenes, precision):
         print(f"  start {start:7.1f} deg: rel L2 {e:.2e}, normwise {normwise:.2e}")
         errs.append(e)
         assert normwise <= 1e-6
-    print(f"{precision}: max {max(errs):.2e}, median {np.median(errs):.2e}")
+    print(f"{precision} fused={fused}: max {max(errs):.2e}, median {np.median(errs):.2e}")
     assert max(errs) <= BOUND[precision]
