@@ -208,6 +208,16 @@ int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int 
                        int *fwd_region_tiles, int *bp_series_order, double *bp_series_err_bins,
                        int *fused_fft_ratio);
 
+/* The forward's adaptive spectral stage (fused kernels, precision other than
+ * exact ranges): the transforms run in FP32, and the rows whose cancellation
+ * factor kappa = sqrt(N ||h||^2 / ||X||^2) exceeds a threshold (||h|| the
+ * profile row's norm, ||X|| its band's; SARHIP_FWD_KAPPA at plan creation,
+ * default 2, 0 turns it off) are transformed again in FP64.  *kappa receives
+ * the threshold (0 when the stage is not adaptive); with `rows` non-NULL,
+ * *rows receives the number of rows of the most recent forward that were
+ * redone (waits for the device).  No counterpart in the reference. */
+int sarhip_plan_forward_fp64_rows(sarhip_plan_t plan, double *kappa, int *rows);
+
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
  * (or re-enabling) clears the log; while enabled every forward / adjoint call
  * records its stage boundaries without synchronising.  sarhip_plan_timing

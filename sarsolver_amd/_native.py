@@ -48,7 +48,7 @@ class SarCalculationInfo(ctypes.Structure):
 EXPORTED_SYMBOLS = (
     "forward_evaluate", "adjoint_evaluate", "sarhip_plan_create", "sarhip_plan_destroy",
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
-    "sarhip_plan_set_timing",
+    "sarhip_plan_set_timing", "sarhip_plan_forward_fp64_rows",
     "sarhip_plan_timing", "sarhip_plan_kernel_timing", "sarhip_last_error", "sarhip_version",
     "sarhip_resample", "sarhip_plan_create_device_positions", "sarhip_plan_create_grid",
     "sarhip_grid_positions", "sarhip_baseband_values", "sarhip_decode_signal",
@@ -97,6 +97,8 @@ def load_library() -> ctypes.CDLL:
         ip = ctypes.POINTER(ctypes.c_int)
         L.sarhip_plan_layout.argtypes = [vp, ip, ip, ip, ip, ip, dp, ip, ip, dp, ip]
         L.sarhip_plan_layout.restype = ctypes.c_int
+        L.sarhip_plan_forward_fp64_rows.argtypes = [vp, dp, ip]
+        L.sarhip_plan_forward_fp64_rows.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
         L.sarhip_plan_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
@@ -391,6 +393,10 @@ class SarPlan:
                        "fwd_pulses_per_group": vals[4].value, "series_err_bins": err.value,
                        "fwd_region_tiles": reg.value, "bp_series_order": bp_ord.value,
                        "bp_series_err_bins": bp_err.value, "fused_fft_ratio": fused.value}
+        kappa = ctypes.c_double()
+        check(L.sarhip_plan_forward_fp64_rows(handle, ctypes.byref(kappa), None),
+              "sarhip_plan_forward_fp64_rows")
+        self.layout["fwd_fft_kappa"] = kappa.value
         self.effective_precision = self.layout["precision"]
 
     @property
@@ -465,6 +471,15 @@ class SarPlan:
         """Start (and clear) or stop the per-call stage event log."""
         check(load_library().sarhip_plan_set_timing(self._handle, int(bool(enable))),
               "sarhip_plan_set_timing")
+
+    def forward_fp64_rows(self) -> int:
+        """Rows of the most recent forward whose spectral stage was redone in
+        FP64 (cancellation factor above ``layout['fwd_fft_kappa']``; 0 when
+        that stage is not adaptive).  Waits for the device."""
+        rows = ctypes.c_int()
+        check(load_library().sarhip_plan_forward_fp64_rows(self._handle, None, ctypes.byref(rows)),
+              "sarhip_plan_forward_fp64_rows")
+        return rows.value
 
     def timing(self, which: str):
         """Mean device ms per logged call of ``which`` ('forward' or 'adjoint'):

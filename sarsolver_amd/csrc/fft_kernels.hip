@@ -423,17 +423,28 @@ struct DownBatch {
 // rounding inside the transforms, relative to the whole energy, is amplified
 // as much: about 8e-6 of the band there.  In FP64 what is left is the FP32
 // rounding of the profile itself.
+//
+// Cancelling rows are found as they are transformed: the FP32 transforms'
+// rounding, relative to the band, grows with kappa = sqrt(N ||h||^2 /
+// ||X||^2) (||h|| the row's norm, ||X|| the band's before the weights; about
+// 0.8 for a dense random scene, 0.4 to 120 over the C4 looks).  With `redo`
+// the FP32 kernel sets redo[p] when kappa^2 > kappa2 (the sums in FP64, so
+// the test is invariant under power-of-two scalings of the input), and the
+// FP64 kernel, launched after it with the same `redo`, transforms those rows
+// only and overwrites their band.
 template <int N, int Q, typename C>
 __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
     const float2 *__restrict__ prof, const int2 *__restrict__ spans, const float2 *__restrict__ W,
     const float2 *__restrict__ wfac, int U, int centred, const C *__restrict__ tables,
-    float2 *__restrict__ Y) {
+    float2 *__restrict__ Y, int *__restrict__ redo, double kappa2) {
     using Bt = DownBatch<N, Q, C>;
     extern __shared__ double2 lds64[];
     C *lds = reinterpret_cast<C *>(lds64);
     constexpr int SZ = N + N / 16;
     constexpr int PER = (N + Bt::NT - 1) / Bt::NT;
     const int p = blockIdx.x;
+    // the FP64 pass of the adaptive mode: only the rows the FP32 pass flagged
+    if (Bt::kF64 && redo && redo[p] == 0) return;
     const int tid = threadIdx.x, g = tid / Bt::T, gt = tid % Bt::T;
     const int nw = U * N;
     const Twiddles<C> tw = load_twiddles<N>(lds + Q * SZ, tables, nw);
@@ -441,8 +452,11 @@ __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
     const int2 sp = spans ? spans[p] : make_int2(0, nw);
     if (sp.y < 0) {  // poisoned row (see the merge)
         for (int k = tid; k < N; k += Bt::NT) Y[(size_t)p * N + k] = make_float2(NAN, NAN);
+        if (!Bt::kF64 && redo && tid == 0) redo[p] = 0;
         return;
     }
+    constexpr bool kMeasure = !Bt::kF64;
+    double e_row = 0.0;  // this thread's part of ||h||^2 (kMeasure with redo)
     C acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) acc[q] = cmake<C>(0.0, 0.0);
@@ -462,8 +476,12 @@ __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
             if (any) {
                 load_phases<Q>(row + n, v);
 #pragma unroll
-                for (int q = 0; q < Q; ++q)
+                for (int q = 0; q < Q; ++q) {
                     if (!in[q]) v[q] = make_float2(0.f, 0.f);
+                    if (kMeasure && redo)
+                        e_row = fma((double)v[q].x, (double)v[q].x,
+                                    fma((double)v[q].y, (double)v[q].y, e_row));
+                }
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; ++q) v[q] = make_float2(0.f, 0.f);
@@ -515,13 +533,37 @@ __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
         __syncthreads();
     }
     const C wp = cmake<C>(wfac[p].x, wfac[p].y);
+    double e_band = 0.0;  // this thread's part of ||X||^2 (kMeasure with redo)
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int k = tid + Bt::NT * q;
         if (k >= N) break;
+        if (kMeasure && redo)
+            e_band = fma((double)acc[q].x, (double)acc[q].x,
+                         fma((double)acc[q].y, (double)acc[q].y, e_band));
         const int dst = centred ? (k + N / 2) % N : k;
         const C y = cmul(cmul(cmake<C>(W[k].x, W[k].y), acc[q]), wp);
         Y[(size_t)p * N + dst] = make_float2((float)y.x, (float)y.y);
+    }
+    if (kMeasure && redo) {
+        // block sums in a fixed order: wave sums by butterfly, then wave 0
+        // adds the waves' sums from LDS (free after the last batch's barrier)
+        for (int off = 32; off > 0; off >>= 1) {
+            e_row += __shfl_xor(e_row, off, 64);
+            e_band += __shfl_xor(e_band, off, 64);
+        }
+        double2 *red = lds64;
+        const int lane = tid & 63, wave = tid >> 6;
+        if (lane == 0) red[wave] = make_double2(e_row, e_band);
+        __syncthreads();
+        if (tid == 0) {
+            double r = 0.0, b = 0.0;
+            for (int w = 0; w < Bt::NT / 64; ++w) {
+                r += red[w].x;
+                b += red[w].y;
+            }
+            redo[p] = (double)N * r > kappa2 * b ? 1 : 0;
+        }
     }
 }
 
@@ -713,13 +755,14 @@ hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const flo
 
 template <int N, int Q, typename C>
 hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac, int U,
-                int P, int centred, const C *tables, float2 *Y, hipStream_t s) {
+                int P, int centred, const C *tables, float2 *Y, int *redo, double kappa2,
+                hipStream_t s) {
     const size_t lds = fused_lds_bytes(N, U, Q, sizeof(C));
     hipError_t e = hipFuncSetAttribute((const void *)fft_down_kernel<N, Q, C>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    fft_down_kernel<N, Q, C><<<P, DownBatch<N, Q, C>::NT, lds, s>>>(prof, spans, W, wfac, U,
-                                                                     centred, tables, Y);
+    fft_down_kernel<N, Q, C><<<P, DownBatch<N, Q, C>::NT, lds, s>>>(
+        prof, spans, W, wfac, U, centred, tables, Y, redo, kappa2);
     return hipGetLastError();
 }
 
@@ -728,27 +771,30 @@ hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const fl
 // selected by fused_fft_phases and are not built.
 template <int N, int Q, typename C>
 hipError_t down_fit(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
-                    int U, int P, int centred, const C *tables, float2 *Y, hipStream_t s) {
+                    int U, int P, int centred, const C *tables, float2 *Y, int *redo,
+                    double kappa2, hipStream_t s) {
     constexpr size_t need = sizeof(C) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + 512);
     if constexpr (need <= kMaxLds)
-        return down<N, Q, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s);
+        return down<N, Q, C>(prof, spans, W, wfac, U, P, centred, tables, Y, redo, kappa2, s);
     else
         return hipErrorInvalidValue;
 }
 
 template <typename C>
 hipError_t down_n(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
-                  int N, int U, int P, int Q, int centred, const C *tables, float2 *Y,
-                  hipStream_t s) {
-#define DOWN_CASE(n)                                                                             \
-    if (N == n) {                                                                                \
-        if (Q == 8) return down_fit<n, 8, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        if (Q == 4) return down_fit<n, 4, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        if (Q == 2) return down_fit<n, 2, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s); \
-        return down_fit<n, 1, C>(prof, spans, W, wfac, U, P, centred, tables, Y, s);             \
+                  int N, int U, int P, int Q, int centred, const C *tables, float2 *Y, int *redo,
+                  double kappa2, hipStream_t s) {
+#define DOWN_ARGS prof, spans, W, wfac, U, P, centred, tables, Y, redo, kappa2, s
+#define DOWN_CASE(n)                                        \
+    if (N == n) {                                           \
+        if (Q == 8) return down_fit<n, 8, C>(DOWN_ARGS);    \
+        if (Q == 4) return down_fit<n, 4, C>(DOWN_ARGS);    \
+        if (Q == 2) return down_fit<n, 2, C>(DOWN_ARGS);    \
+        return down_fit<n, 1, C>(DOWN_ARGS);                \
     }
     SARHIP_FFT_SIZES(DOWN_CASE)
 #undef DOWN_CASE
+#undef DOWN_ARGS
     return hipErrorInvalidValue;
 }
 
@@ -799,11 +845,19 @@ hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj
 
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables,
-                           const double2 *tables64, float2 *Y, bool sparse, hipStream_t s) {
-    const bool fp64 = tables64 != nullptr;
-    const int Q = fused_fft_phases(N, U, false, sparse, fp64);
-    if (fp64) return down_n<double2>(prof, spans, W, wfac, N, U, P, Q, centred, tables64, Y, s);
-    return down_n<float2>(prof, spans, W, wfac, N, U, P, Q, centred, tables, Y, s);
+                           const double2 *tables64, float2 *Y, bool sparse, int *redo,
+                           double kappa2, hipStream_t s) {
+    const int Q64 = fused_fft_phases(N, U, false, sparse, true);
+    if (tables64 && !redo)
+        return down_n<double2>(prof, spans, W, wfac, N, U, P, Q64, centred, tables64, Y, nullptr,
+                               0.0, s);
+    const int Q = fused_fft_phases(N, U, false, sparse, false);
+    const bool adaptive = tables64 && redo;
+    hipError_t e = down_n<float2>(prof, spans, W, wfac, N, U, P, Q, centred, tables, Y,
+                                  adaptive ? redo : nullptr, kappa2, s);
+    if (e != hipSuccess || !adaptive) return e;
+    return down_n<double2>(prof, spans, W, wfac, N, U, P, Q64, centred, tables64, Y, redo, kappa2,
+                           s);
 }
 
 bool spectral_filter_supported(int n, int M) {

@@ -148,10 +148,14 @@ hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj
                          int N, int U, int ld, int P, int centred, const float2 *tables,
                          float2 *prof, bool sparse, hipStream_t s);
 // spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows;
-// sparse: as for launch_fft_up; tables64 non-NULL: transforms and phase sums in FP64
+// sparse: as for launch_fft_up.  tables64 non-NULL and redo NULL: transforms and
+// phase sums in FP64 for every row; both non-NULL: FP32, then FP64 again for the
+// rows whose cancellation factor kappa^2 = N ||h||^2 / ||X||^2 exceeds kappa2
+// (redo [P] receives the flags); tables64 NULL: FP32
 hipError_t launch_fft_down(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                            int N, int U, int P, int centred, const float2 *tables,
-                           const double2 *tables64, float2 *Y, bool sparse, hipStream_t s);
+                           const double2 *tables64, float2 *Y, bool sparse, int *redo,
+                           double kappa2, hipStream_t s);
 // spectral filter of the learned preconditioner (fft_kernels.hip): out =
 // dout . crop_n IFFT2_M(f / M^2 . FFT2_M(pad_M(conj(din) . in))), B images n x n,
 // filter f [M][M] float (ky, kx), work >= B n M complex64, din / dout optional

@@ -128,7 +128,11 @@ struct sarhip_plan_s {
     int fused_U = 0;                                // > 0: fused spectral kernels, N_w = U N
     bool fft_sparse = false;  // adjoint spans cover under half of the valid rows' bins
     float2 *d_fft_tables = nullptr;                 // twiddle tables of the fused kernels
-    double2 *d_fft_tables64 = nullptr;              // the same in FP64 (forward, exact ranges)
+    double2 *d_fft_tables64 = nullptr;              // the same in FP64 (forward: exact
+                                                    // ranges, or the cancelling rows)
+    int *d_fft_redo = nullptr;  // [P] forward rows redone in FP64 (adaptive spectral stage)
+    double fwd_kappa = 0.0, fwd_kappa2 = 0.0;  // > 0: adaptive forward spectral stage
+                                               // (threshold on kappa, its square)
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     SeriesGather *d_series_g = nullptr;  // [num_tiles][Pv] gather records (see SeriesGather)
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
@@ -177,6 +181,7 @@ struct sarhip_plan_s {
         dev_free(d_srec);
         dev_free(d_fft_tables);
         dev_free(d_fft_tables64);
+        dev_free(d_fft_redo);
         dev_free(d_W);
         dev_free(d_Wconj);
         dev_free(d_wfac_fwd);
@@ -986,8 +991,17 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
             for (const double2 &t : tab64) tab.push_back(make_float2((float)t.x, (float)t.y));
             if ((rc = upload((void **)&pl->d_fft_tables, tab.data(), sizeof(float2) * tab.size())))
                 return rc;
-            // exact ranges requested: the forward's transforms in FP64 as well
-            if (pl->requested_precision == SARHIP_PRECISION_FP64 &&
+            // exact ranges requested: the forward's transforms in FP64 as well;
+            // otherwise FP32 transforms, and FP64 again for the rows that cancel
+            // (kappa above SARHIP_FWD_KAPPA, default 2; 0 turns this off)
+            const double kappa = env_double("SARHIP_FWD_KAPPA", 2.0);
+            if (pl->requested_precision != SARHIP_PRECISION_FP64 && kappa > 0.0) {
+                pl->fwd_kappa = kappa;
+                pl->fwd_kappa2 = kappa * kappa;
+                HIP_TRY(hipMalloc((void **)&pl->d_fft_redo, sizeof(int) * std::max<size_t>(P, 1)));
+                HIP_TRY(hipMemset(pl->d_fft_redo, 0, sizeof(int) * std::max<size_t>(P, 1)));
+            }
+            if ((pl->requested_precision == SARHIP_PRECISION_FP64 || pl->fwd_kappa2 > 0.0) &&
                 (rc = upload((void **)&pl->d_fft_tables64, tab64.data(),
                              sizeof(double2) * tab64.size())))
                 return rc;
@@ -1198,6 +1212,24 @@ int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int 
     return SARHIP_OK;
 }
 
+int sarhip_plan_forward_fp64_rows(sarhip_plan_t plan, double *kappa, int *rows) {
+    if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
+    const bool adaptive = plan->fused_U > 0 && plan->d_fft_redo;
+    if (kappa) *kappa = adaptive ? plan->fwd_kappa : 0.0;
+    if (rows) {
+        *rows = 0;
+        if (adaptive) {
+            DeviceGuard g(plan->device);
+            std::vector<int> f(plan->P);
+            HIP_TRY(hipDeviceSynchronize());
+            HIP_TRY(hipMemcpy(f.data(), plan->d_fft_redo, sizeof(int) * plan->P,
+                              hipMemcpyDeviceToHost));
+            for (const int v : f) *rows += v != 0;
+        }
+    }
+    return SARHIP_OK;
+}
+
 int sarhip_plan_set_timing(sarhip_plan_t plan, int enable) {
     if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
     plan->timing = enable != 0;
@@ -1283,7 +1315,7 @@ int sarhip_forward(sarhip_plan_t pl, const void *amps, void *phase_history, int 
         HIP_TRY(launch_fft_down(pl->d_prof, spans ? pl->d_fspan : nullptr, pl->d_W, pl->d_wfac_fwd,
                                 (int)pl->N, pl->fused_U, (int)pl->P, order == SARHIP_ORDER_CENTRED,
                                 pl->d_fft_tables, pl->d_fft_tables64, (float2 *)phase_history,
-                                spans && pl->fft_sparse, s));
+                                spans && pl->fft_sparse, pl->d_fft_redo, pl->fwd_kappa2, s));
         if ((rc = record(pl, 2, s))) return rc;
         if ((rc = record(pl, 3, s))) return rc;
         return SARHIP_OK;

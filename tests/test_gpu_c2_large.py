@@ -53,6 +53,8 @@ def c2(dev, oracle_lib):
     y = pl.forward(xs)                              # the scene's phase history
     X = pl.adjoint(y).cpu().numpy()
     Yd = pl.forward(torch.from_numpy(xdense.astype(np.complex64)).to(dev)).cpu().numpy()
+    # the dense image does not cancel (kappa ~0.8): no row needs the FP64 pass
+    dense_fp64_rows = pl.forward_fp64_rows()
     Ys = y.cpu().numpy()
     del pl
     torch.cuda.empty_cache()
@@ -90,7 +92,7 @@ def c2(dev, oracle_lib):
                                interp="linear", num_threads=8)
     print("oracle done", flush=True)
     return dict(w=w, lay=lay, X=X[pix], X_or=X_or, Yd=Yd[rows], Yd_or=Yd_or,
-                Ys=Ys[rows], Ys_or=Ys_or)
+                Ys=Ys[rows], Ys_or=Ys_or, dense_fp64_rows=dense_fp64_rows)
 
 
 def test_c2_plan_layout_is_the_benchmarked_one(c2):
@@ -99,6 +101,11 @@ def test_c2_plan_layout_is_the_benchmarked_one(c2):
     print(f"c2 layout: {lay}")
     assert got == BENCH_LAYOUT
     assert lay["series_err_bins"] <= 1e-9 and lay["bp_series_err_bins"] <= 1e-9
+    # the adaptive spectral stage is on, and the benchmarked dense forward
+    # runs every row in FP32
+    assert lay["fwd_fft_kappa"] == 2.0
+    print(f"c2 dense forward: {c2['dense_fp64_rows']} rows redone in FP64")
+    assert c2["dense_fp64_rows"] == 0
 
 
 def test_c2_adjoint_pixel_subset_against_oracle(c2):
