@@ -614,9 +614,8 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 //
 // Windows are written whole (zeros included) to `partial` as their integer
 // sums, with their base bins and max |component| (in their units), then
-// merged per pulse by fp_reduce_kernel in 64-bit integers: no global atomics
-// on the main path, no rounding between the windows and the row, and a result
-// that does not depend on scheduling.
+// merged per pulse by fp_reduce_kernel in FP64 in a fixed order: no global
+// atomics on the main path, and a result that does not depend on scheduling.
 // ---------------------------------------------------------------------------
 // Toward zero (one v_cvt_i32_f32).  Its error is odd in v, so bin sums of
 // contributions with mixed signs carry no bias; rounding halves always up
@@ -1136,51 +1135,46 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     if (tid < np) k.wbases[(size_t)(j0 + tid) * NR + region] = wb;
 }
 
-// One workgroup per valid pulse: merge the pulse's integer window partials
-// into its profile row, with no atomics on the sums.  The covered span of bins
-// is cut into 64-bin blocks, and every non-zero window is listed under each
-// block it meets (a counting sort in LDS).  Wave w takes blocks w, w + 8, ...:
-// lane l sums bin 64 b + l over the block's windows in 64-bit integer
-// registers, a window's part of the block being one coalesced load, kRedBatch
-// windows per batch; every partial is read once.
+// One workgroup per valid pulse: merge the pulse's window partials into its
+// profile row, with no atomics.  The span [lo, hi) of the row's non-zero
+// windows is cut into 64-bin blocks; wave w takes blocks w, w + 8, ...: it
+// lists the windows that meet the block, in window order (a ballot over 64
+// windows at a time, each lane testing one window's block range), and lane l
+// sums bin 64 b + l over them, kRedBatch loads in flight (a window's part of
+// the block is one coalesced load).  Every partial is read once.
 //
-// Sums are at the unit 2^-E, E = e_min + H, with e_min the coarsest scale
-// exponent among the row's non-zero windows and H = clz(NR) - 1: a window sum
-// s (|s| < 2^31 units mm 2^-e, mm = M 2^-24 with the integer M < 2^24) enters
-// as s M 2^(E - e - 24), formed in integers (a 64-bit product, then a shift;
-// right shifts round), below 2^(31 + H), so a sum over the NR windows stays
-// below 2^62; the rounding is at most 2^-(32 + H) of the coarsest window's
-// bound.  Each bin is converted once (the row's only further rounding) and
+// A window sum s (an integer, |s| < 2^31, at the window's fixed-point scale
+// m 2^e) enters as s * ((1 / m) 2^-e); the factor is exact in FP64, so each
+// product and each sum rounds at 2^-53 relative, in a fixed order (the result
+// does not depend on scheduling), and a power-of-two scaling of the image
+// scales every term and sum exactly.  Each bin is rounded once to FP32 and
 // stored into the row (span path) or added to it (rows cleared before the
 // scatter).  The span is merged in passes of at most min(N_w, 64 kRedBlocks)
 // bins, so that no two bins of one pass share a row bin when it is longer
-// than the row.
+// than the row.  (The first exact merge summed 64-bit integers at a row-wide
+// unit: about 35 VALU instructions per window and block against 11 here.)
 constexpr int kRedThreads = 512;
-constexpr int kRedBatch = 4;                   // windows per batch
+constexpr int kRedWaves = kRedThreads / 64;
+constexpr int kRedBatch = 8;                   // loads in flight per lane
+constexpr int kRedList = 128;                  // per-wave list capacity
 constexpr int kRedBlocks = 4 * kRedThreads;    // 64-bin blocks per pass
-__device__ __forceinline__ long long red_at_unit(const int x, const int2 f) {
-    const long long v = (long long)x * (long long)f.x;
-    if (f.y >= 0) return v << f.y;
-    if (f.y <= -63) return 0;
-    return (v + (1ll << (-f.y - 1))) >> -f.y;
-}
 __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelParams k) {
-    // [NR] window bases (double, NaN for a window that is all zero), [NR]
-    // window factors {M, shift}, then the block lists (window indices)
-    extern __shared__ double sb[];
-    __shared__ int cnt[kRedBlocks + 1];  // counts, then list offsets
-    __shared__ int cur[kRedBlocks];      // fill cursors
-    __shared__ double red_lo[kRedThreads], red_hi[kRedThreads];
-    __shared__ int red_e[kRedThreads];
+    // [NR] window factors (1 / m) 2^-e (double), then [NR] window offsets from
+    // lo in bins (int, -1 for a window that is all zero)
+    extern __shared__ double sc[];
+    // per-wave lists: {window offset r, index of its bin 0 less r}, factor
+    __shared__ int2 lists[kRedWaves][kRedList];
+    __shared__ double listf[kRedWaves][kRedList];
+    __shared__ double red_lo[kRedWaves], red_hi[kRedWaves];
     const int j = blockIdx.x;
     const int p = k.rows[j];
     const int NR = k.num_regions, win = k.win;
-    int2 *su = (int2 *)(sb + NR);
-    int *list = (int *)(su + NR);
+    int *sr = (int *)(sc + NR);
     const double *b = k.wbases + (size_t)j * NR;
     const float *wm = k.wmax + (size_t)j * NR;
     const int2 *part = k.partial + (size_t)j * NR * win;
     const int tid = (int)threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
     // a row the scatter poisoned: no span, NaN output row (flag cleared here)
     __shared__ int s_poison;
     if (tid == 0) s_poison = k.fwd_spans ? k.fpoison[p] : 0;
@@ -1193,31 +1187,29 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
         return;
     }
     double lo = INFINITY, hi = -INFINITY;
-    int emin = INT_MAX;
     for (int t = tid; t < NR; t += kRedThreads) {
         const double bt = b[t];
-        const bool live = wm[t] > 0.f;  // a window with a non-zero sum
-        sb[t] = live ? bt : NAN;
-        if (live) {
+        if (wm[t] > 0.f) {  // a window with a non-zero sum
             lo = fmin(lo, bt);
             hi = fmax(hi, bt);
-            emin = min(emin, fix_scale(k.region_l1[t]).e);
         }
     }
-    red_lo[tid] = lo;
-    red_hi[tid] = hi;
-    red_e[tid] = emin;
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+    }
+    if (lane == 0) {
+        red_lo[wave] = lo;
+        red_hi[wave] = hi;
+    }
     __syncthreads();
-    for (int s = kRedThreads / 2; s > 0; s >>= 1) {
-        if (tid < s) {
-            red_lo[tid] = fmin(red_lo[tid], red_lo[tid + s]);
-            red_hi[tid] = fmax(red_hi[tid], red_hi[tid + s]);
-            red_e[tid] = min(red_e[tid], red_e[tid + s]);
-        }
-        __syncthreads();
-    }
     lo = red_lo[0];
-    hi = red_hi[0] + (double)win;
+    hi = red_hi[0];
+    for (int w = 1; w < kRedWaves; ++w) {
+        lo = fmin(lo, red_lo[w]);
+        hi = fmax(hi, red_hi[w]);
+    }
+    hi += (double)win;
     // With fwd_spans the covered bins [lo, hi) are written (stored) once each,
     // and their span is recorded for the spectral stage; the rest of the row
     // is not touched (a span longer than the row wraps onto itself: the row is
@@ -1231,15 +1223,16 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
         k.fspan[p] = empty ? make_int2(0, 0)
                            : full ? make_int2(0, k.nw) : make_int2(bin_index(k, lo, 1), (int)(hi - lo));
     if (empty) return;
-    const int E = red_e[0] + __clz(NR) - 1;  // the accumulation unit 2^-E
-    const double unit = ldexp(1.0, -E);
+    // window bases are integers (bins), so the offsets are exact
     for (int t = tid; t < NR; t += kRedThreads) {
-        int2 f = make_int2(0, 0);
-        if (!isnan(sb[t])) {
+        const bool live = wm[t] > 0.f;
+        double c = 0.0;
+        if (live) {
             const FixScale fs = fix_scale(k.region_l1[t]);
-            f = make_int2((int)ldexpf(fs.inv_m, 24), E - fs.e - 24);
+            c = ldexp((double)fs.inv_m, -fs.e);
         }
-        su[t] = f;
+        sc[t] = c;
+        sr[t] = live ? (int)(b[t] - lo) : -1;
     }
     float2 *row = k.prof + (size_t)p * k.nw;
     if (full && store) {
@@ -1249,94 +1242,63 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     __syncthreads();
     const int S = (int)(hi - lo);  // bins of the span
     const int W = min(k.nw, 64 * kRedBlocks);
-    const int lane = tid & 63, wave = tid >> 6;
-    constexpr int kWaves = kRedThreads / 64;
+    int2 *list = lists[wave];
+    double *listf_w = listf[wave];
+    const unsigned long long below = (1ull << lane) - 1ull;  // lanes under this one
     for (int c0 = 0; c0 < S; c0 += W) {
         const int len = min(W, S - c0);
         const int nb = (len + 63) / 64;
-        // blocks [first, last] of this pass that window t meets
-        auto blocks = [&](const int t, int &first, int &last) -> bool {
-            const double bt = sb[t];
-            if (isnan(bt)) return false;
-            const int r0 = (int)(bt - lo) - c0;  // the window's first bin in the pass
-            if (r0 >= len || r0 + win <= 0) return false;
-            first = max(r0, 0) >> 6;
-            last = min(r0 + win - 1, len - 1) >> 6;
-            return true;
-        };
-        for (int e = tid; e <= nb; e += kRedThreads) cnt[e] = 0;
-        __syncthreads();
-        for (int t = tid; t < NR; t += kRedThreads) {
-            int f, l;
-            if (blocks(t, f, l))
-                for (int q = f; q <= l; ++q) atomicAdd(&cnt[q], 1);
-        }
-        __syncthreads();
-        // exclusive scan of cnt[0, nb) (4 consecutive entries per thread), the
-        // total in cnt[nb]; the fill cursors start at the offsets
-        {
-            const int e0 = 4 * tid;
-            int v4[4], sum = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                v4[q] = e0 + q < nb ? cnt[e0 + q] : 0;
-                sum += v4[q];
-            }
-            red_e[tid] = sum;
-            __syncthreads();
-            for (int off = 1; off < kRedThreads; off <<= 1) {
-                const int add = tid >= off ? red_e[tid - off] : 0;
-                __syncthreads();
-                red_e[tid] += add;
-                __syncthreads();
-            }
-            int run = red_e[tid] - sum;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (e0 + q < nb) {
-                    cnt[e0 + q] = run;
-                    cur[e0 + q] = run;
+        for (int blk = wave; blk < nb; blk += kRedWaves) {
+            const int ab = (c0 >> 6) + blk;  // the block's index from lo
+            const int rel = c0 + 64 * blk + lane;  // the lane's bin from lo
+            double ar = 0.0, ai = 0.0;
+            int n = 0;  // listed windows (wave-uniform)
+            for (int t0 = 0; t0 < NR; t0 += 64) {
+                const int t = t0 + lane;
+                bool meet = false;
+                int r = -1;
+                if (t < NR) {
+                    r = sr[t];
+                    meet = r >= 0 && (r >> 6) <= ab && ((r + win - 1) >> 6) >= ab;
                 }
-                run += v4[q];
-            }
-            if (tid == kRedThreads - 1) cnt[nb] = red_e[tid];
-        }
-        __syncthreads();
-        for (int t = tid; t < NR; t += kRedThreads) {
-            int f, l;
-            if (blocks(t, f, l))
-                for (int q = f; q <= l; ++q) list[atomicAdd(&cur[q], 1)] = t;
-        }
-        __syncthreads();
-        for (int blk = wave; blk < nb; blk += kWaves) {
-            const int nrel = 64 * blk + lane;  // the lane's bin in the pass
-            const int beg = cnt[blk], end = cnt[blk + 1];
-            long long ar = 0, ai = 0;
-            for (int i = beg; i < end; i += kRedBatch) {
-                int2 v[kRedBatch], f[kRedBatch];
+                const unsigned long long mask = __ballot(meet);
+                if (meet) {
+                    const int at = n + __popcll(mask & below);
+                    list[at] = make_int2(r, t * win - r);
+                    listf_w[at] = sc[t];
+                }
+                n += __popcll(mask);
+                if (n > kRedList - 64 || (t0 + 64 >= NR && n > 0)) {
+                    for (int i = 0; i < n; i += kRedBatch) {
+                        int2 v[kRedBatch];
+                        double f[kRedBatch];
 #pragma unroll
-                for (int q = 0; q < kRedBatch; ++q) {
-                    v[q] = make_int2(0, 0);
-                    f[q] = make_int2(0, 0);
-                    if (i + q < end) {
-                        const int t = list[i + q];
-                        const int o = c0 + nrel - (int)(sb[t] - lo);
-                        f[q] = su[t];
-                        if (o >= 0 && o < win) v[q] = part[(size_t)t * win + o];
+                        for (int q = 0; q < kRedBatch; ++q) {
+                            v[q] = make_int2(0, 0);
+                            f[q] = 0.0;
+                            if (i + q < n) {
+                                const int2 e = list[i + q];
+                                const int o = rel - e.x;
+                                f[q] = listf_w[i + q];
+                                if (o >= 0 && o < win) v[q] = part[e.y + rel];
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < kRedBatch; ++q) {
+                            ar = fma((double)v[q].x, f[q], ar);
+                            ai = fma((double)v[q].y, f[q], ai);
+                        }
                     }
-                }
-#pragma unroll
-                for (int q = 0; q < kRedBatch; ++q) {
-                    ar += red_at_unit(v[q].x, f[q]);
-                    ai += red_at_unit(v[q].y, f[q]);
+                    n = 0;
                 }
             }
+            const int nrel = 64 * blk + lane;  // the lane's bin in the pass
             if (nrel < len) {
-                const int n = bin_index(k, lo + (double)(c0 + nrel), 1);
-                float2 r = (full || !store) ? row[n] : make_float2(0.f, 0.f);
-                r.x += (float)((double)ar * unit);
-                r.y += (float)((double)ai * unit);
-                row[n] = r;
+                const int nn = bin_index(k, lo + (double)(c0 + nrel), 1);
+                float2 r = (full || !store) ? row[nn] : make_float2(0.f, 0.f);
+                r.x += (float)ar;
+                r.y += (float)ai;
+                row[nn] = r;
             }
         }
         __threadfence_block();
@@ -1631,9 +1593,9 @@ hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s) {
 }
 
 size_t fp_reduce_lds(int num_regions, int win) {
+    (void)win;
     const size_t nr = (size_t)std::max(num_regions, 0);
-    const size_t per = sizeof(double) + sizeof(int2) + sizeof(int) * ((size_t)std::max(win - 1, 0) / 64 + 2);
-    return nr * per;
+    return nr * (sizeof(double) + sizeof(int));
 }
 
 hipError_t launch_fp_reduce(const KernelParams &k, hipStream_t s) {

@@ -14,8 +14,8 @@ precision="series" keeps the phasor in FP32 and runs the spectral stage in
 FP32, redoing in FP64 the rows whose cancellation factor kappa =
 sqrt(N ||h||^2 / ||X||^2) exceeds 2 (0.4 to 120 over these looks, ~0.8 for a
 dense random image); with FP32 transforms throughout its worst look was
-1.12e-5.  Both modes are held to 1e-5 on every scene; both merge the windows
-exactly (64-bit integers), and both are held to a normwise error
+1.12e-5.  Both modes are held to 1e-5 on every scene; both merge the windows'
+integer sums in FP64, and both are held to a normwise error
 ||dY|| / (||A|| ||x||) <= 1e-6, ||A||^2 = P N for unit weights.
 """
 import numpy as np
