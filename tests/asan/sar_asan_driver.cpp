@@ -138,6 +138,13 @@ void plans(const Problem &p, int interp, int precision) {
     CHECK(sarhip_plan_layout(pl, &prec, &mono, &valid, &win, &G, &se, &rt, &so, &bse, &fu) ==
               SARHIP_OK && mono == 1 && valid == (int)p.P,
           "plan_layout");
+    // adaptive spectral stage: a threshold unless exact ranges (every row in
+    // FP64) or the rocFFT path; the row count reads the plan's flags back
+    double kappa = -1.0;
+    int redone = -1;
+    CHECK(sarhip_plan_forward_fp64_rows(pl, &kappa, &redone) == SARHIP_OK && kappa >= 0.0 &&
+              redone >= 0 && redone <= (int)p.P && (precision != 0 || kappa == 0.0),
+          "forward_fp64_rows");
     CHECK(sarhip_plan_set_timing(pl, 1) == SARHIP_OK, "set_timing");
     std::vector<float> Yt, Xt;
     run_plan(pl, p, Yt, Xt);
