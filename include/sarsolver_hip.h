@@ -218,6 +218,16 @@ int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int 
  * redone (waits for the device).  No counterpart in the reference. */
 int sarhip_plan_forward_fp64_rows(sarhip_plan_t plan, double *kappa, int *rows);
 
+/* Whether the backprojection reads demodulated pair rows: after the spectral
+ * stage a pass stores, over the bins the gather can read, (h[n], h[n+1] -
+ * h[n]) e^{-2 pi i rho (n + 1/2)} (rho = k_c / (2 pi by_dx), the phase in
+ * revolutions per bin), and the gather interpolates with two FMAs and forms
+ * each pixel's phase rho f + phi' in FP32 (f the bin fraction).  Enabled for
+ * the monostatic planar series plans with fused spectral stages when rho is
+ * an FP32 number (to 2^-34) of magnitude at most 2.5; SARHIP_BP_PAIRS=0 at
+ * plan creation turns it off.  No counterpart in the reference. */
+int sarhip_plan_gather_pairs(sarhip_plan_t plan, int *enabled);
+
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
  * (or re-enabling) clears the log; while enabled every forward / adjoint call
  * records its stage boundaries without synchronising.  sarhip_plan_timing

@@ -48,7 +48,7 @@ class SarCalculationInfo(ctypes.Structure):
 EXPORTED_SYMBOLS = (
     "forward_evaluate", "adjoint_evaluate", "sarhip_plan_create", "sarhip_plan_destroy",
     "sarhip_forward", "sarhip_adjoint", "sarhip_plan_info", "sarhip_plan_layout",
-    "sarhip_plan_set_timing", "sarhip_plan_forward_fp64_rows",
+    "sarhip_plan_set_timing", "sarhip_plan_forward_fp64_rows", "sarhip_plan_gather_pairs",
     "sarhip_plan_timing", "sarhip_plan_kernel_timing", "sarhip_last_error", "sarhip_version",
     "sarhip_resample", "sarhip_plan_create_device_positions", "sarhip_plan_create_grid",
     "sarhip_grid_positions", "sarhip_baseband_values", "sarhip_decode_signal",
@@ -99,6 +99,8 @@ def load_library() -> ctypes.CDLL:
         L.sarhip_plan_layout.restype = ctypes.c_int
         L.sarhip_plan_forward_fp64_rows.argtypes = [vp, dp, ip]
         L.sarhip_plan_forward_fp64_rows.restype = ctypes.c_int
+        L.sarhip_plan_gather_pairs.argtypes = [vp, ip]
+        L.sarhip_plan_gather_pairs.restype = ctypes.c_int
         L.sarhip_plan_set_timing.argtypes = [vp, ctypes.c_int]
         L.sarhip_plan_set_timing.restype = ctypes.c_int
         L.sarhip_plan_timing.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
@@ -397,6 +399,9 @@ class SarPlan:
         check(L.sarhip_plan_forward_fp64_rows(handle, ctypes.byref(kappa), None),
               "sarhip_plan_forward_fp64_rows")
         self.layout["fwd_fft_kappa"] = kappa.value
+        pairs = ctypes.c_int()
+        check(L.sarhip_plan_gather_pairs(handle, ctypes.byref(pairs)), "sarhip_plan_gather_pairs")
+        self.layout["bp_pairs"] = bool(pairs.value)
         self.effective_precision = self.layout["precision"]
 
     @property

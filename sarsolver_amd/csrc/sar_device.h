@@ -65,7 +65,9 @@ static_assert(sizeof(SeriesMono) == 64, "SeriesMono must be 64 bytes");
 // moved by a multiple of nw into the padded row (as the linear gather needs),
 // theta = rho u + phi with rho = kc_rev / by_dx and phi = ct - rho cu.
 struct SeriesGather {
-    double bx, by, k1, k2, k3, cu, phi, pad;
+    double bx, by, k1, k2, k3, cu, phi;
+    float phi_pair;  // phi - rho / 2 reduced to [-1/2, 1/2) (demodulated pair rows)
+    float pad;
 };
 static_assert(sizeof(SeriesGather) == 64, "SeriesGather must be 64 bytes");
 static_assert(sizeof(SeriesBi) == 128, "SeriesBi must be 128 bytes");

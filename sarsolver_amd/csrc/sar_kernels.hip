@@ -52,6 +52,17 @@ __device__ __forceinline__ i32x4 row_pair_rsrc(const float2 *row, const int ld) 
     return r;
 }
 
+// The same over a pair row (16-byte entries, the last one never valid).
+__device__ __forceinline__ i32x4 pair_rsrc(const float4 *row, const int ld) {
+    const unsigned long long a = (unsigned long long)row;
+    i32x4 r;
+    r.x = (int)(unsigned)a;
+    r.y = (int)(((unsigned)(a >> 32) & 0xffffu) | (16u << 16));  // stride 16 bytes
+    r.z = ld - 1;
+    r.w = kBufferWord3;
+    return r;
+}
+
 // Two consecutive profile samples (8-byte aligned) read as one 16-byte load.
 struct alignas(8) Float2x2 {
     float a, b, c, d;
@@ -152,6 +163,43 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float4 *__restrict__ buf
     for (; i < n4; i += stride) buf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (blockIdx.x == 0 && threadIdx.x < bytes - 16 * n4)
         ((unsigned char *)(buf + n4))[threadIdx.x] = 0;
+}
+
+// Demodulated pairs of the adjoint profile rows (the gather's pair path).
+// With u = n + f (n the bin, f in [0, 1)), the gather's sample times its phasor
+//   (h[n] + f (h[n+1] - h[n])) e^{-2 pi i (rho u + phi)}
+//     = (A_n + f D_n) e^{-2 pi i (rho f + phi - rho / 2)}
+// with (A_n, D_n) = (h[n], h[n+1] - h[n]) e^{-2 pi i rho (n + 1/2)}, so the
+// gather forms the pixel phase rho f + phi' in FP32 (|.| < rho / 2 + 1/2)
+// instead of reducing rho u + phi in FP64, and interpolates with two FMAs.
+// Each pair is formed in FP64 from the FP32 row and rounded once; the phasor
+// is sincospi of the phase reduced in FP64.  One workgroup per valid row: the
+// bins of its span (wrapping at N_w; the bins the gather can read, with
+// margin) and their tail copies, as the spectral stage stores them.
+__global__ __launch_bounds__(256) void pair_rows_kernel(const float2 *__restrict__ prof,
+                                                        const int2 *__restrict__ spans,
+                                                        const int *__restrict__ rows, int nw,
+                                                        int ld, double rho,
+                                                        float4 *__restrict__ pairs) {
+    const int p = rows[blockIdx.x];
+    const int2 sp = spans[p];  // first bin in [0, nw), count <= nw
+    const float2 *row = prof + (size_t)p * ld;
+    float4 *out = pairs + (size_t)p * ld;
+    for (int d = (int)threadIdx.x; d < sp.y; d += 256) {
+        int n = sp.x + d;
+        n -= n >= nw ? nw : 0;
+        for (int i = n; i < ld - 1; i += nw) {
+            const float2 a = row[i], b = row[i + 1];
+            double ph = rho * ((double)i + 0.5);
+            ph -= floor(ph);
+            double sn, cs;
+            sincospi(2.0 * ph, &sn, &cs);
+            const double ar = a.x, ai = a.y;
+            const double dr = (double)b.x - ar, di = (double)b.y - ai;
+            out[i] = make_float4((float)(ar * cs + ai * sn), (float)(ai * cs - ar * sn),
+                                 (float)(dr * cs + di * sn), (float)(di * cs - dr * sn));
+        }
+    }
 }
 
 // amps_perm[i] = amps[perm[i]]: pixel amplitudes into tile (Morton) order.
@@ -433,7 +481,7 @@ __device__ __forceinline__ int bin_index(const KernelParams &k, double n, int sp
 // sums in FP32; every 64 pulses the per-pixel FP32 sums are folded into FP64
 // sums kept in LDS (out of registers, for occupancy).
 // ---------------------------------------------------------------------------
-template <int INTERP, bool MONO, int PREC, bool PLANAR, int SORD = 5>
+template <int INTERP, bool MONO, int PREC, bool PLANAR, int SORD = 5, bool PAIRS = false>
 __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
     __shared__ double acc[2 * kPixPerThread][kThreads];
     const PolyRegs pr = make_poly_regs<SORD>();
@@ -464,6 +512,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
     // constants and the row shift (Horner form, phase from u).
     constexpr bool kHG = kShift && MONO && PLANAR && PREC == kPrecSeries && SORD == 3;
     const double rho = k.kc_rev / k.by_dx;
+    const float rho32 = (float)rho;
     int p_next = 0;
     PulseData<MONO> d_next;
     SeriesGather g_next;
@@ -493,7 +542,32 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
         // 16-byte loads of bins (n, n + 1) through a structured buffer over the
         // padded row: indices beyond it read zeros instead of faulting.
         const i32x4 rsrc = row_pair_rsrc(row, k.ld);
-        if constexpr (INTERP == kInterpLinear && kShift) {
+        if constexpr (PAIRS) {
+            // demodulated pair rows (pair_rows_kernel): one 16-byte load of
+            // (A_n, D_n), the phase rho f + phi' in FP32
+            static_assert(kHG, "the pair path reads SeriesGather records");
+            const i32x4 psrc = pair_rsrc(k.bp_pairs + (size_t)p * k.ld, k.ld);
+            u32x4 h[kPixPerThread];
+            float fr[kPixPerThread], sn[kPixPerThread], cs[kPixPerThread];
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, q[m].d2));
+                const double u = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
+                const int n0 = (int)u;
+                fr[m] = (float)__builtin_amdgcn_fract(u);
+                h[m] = (u32x4)llvm_struct_buffer_load_b128(psrc, n0, 0, 0, 0);
+                const float th = fmaf(rho32, fr[m], g.phi_pair);
+                sn[m] = __builtin_amdgcn_sinf(th);
+                cs[m] = __builtin_amdgcn_cosf(th);
+            }
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                const float hr = fmaf(fr[m], __uint_as_float(h[m].z), __uint_as_float(h[m].x));
+                const float hi = fmaf(fr[m], __uint_as_float(h[m].w), __uint_as_float(h[m].y));
+                ar[m] = fmaf(hr, cs[m], fmaf(hi, sn[m], ar[m]));
+                ai[m] = fmaf(hi, cs[m], fmaf(-hr, sn[m], ai[m]));
+            }
+        } else if constexpr (INTERP == kInterpLinear && kShift) {
             // all pixels' bins and phases first, so that their profile loads
             // are in flight together, then interpolation and accumulation
             u32x4 h[kPixPerThread];
@@ -1345,6 +1419,13 @@ hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s
     return hipGetLastError();
 }
 
+hipError_t launch_pair_rows(const float2 *prof, const int2 *spans, const int *rows, int Pv,
+                            int nw, int ld, double rho, float4 *pairs, hipStream_t s) {
+    if (Pv <= 0) return hipSuccess;
+    pair_rows_kernel<<<Pv, 256, 0, s>>>(prof, spans, rows, nw, ld, rho, pairs);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s) {
     const size_t n4 = bytes / 16;
     size_t blocks = (n4 + 255) / 256;
@@ -1447,9 +1528,13 @@ __global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__
     g.k1 = s.ku * 0.5 * s.c;
     g.k2 = s.ku * -0.125 * c2;
     g.k3 = s.ku * 0.0625 * (c2 * s.c);
-    g.pad = 0.0;
+    g.pad = 0.f;
     g.cu = fma(-floor((s.cu - shift_lo) * inv_nw), nwd, s.cu);  // as bp_shift_kernel
     g.phi = fma(-rho, g.cu, s.ct);
+    // the pair path's phase constant: phi reduced to [-1/2, 1/2], less rho / 2
+    // (the pair rows carry e^{-2 pi i rho (n + 1/2)}), so that the pixel phase
+    // rho f + phi_pair stays within rho / 2 + 1/2 of 0
+    g.phi_pair = (float)((g.phi - rint(g.phi)) - 0.5 * rho);
     out[i] = g;
 }
 
@@ -1491,8 +1576,18 @@ static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     switch (k.precision) {
         case kPrecSeries:
             if (I == kInterpLinear && k.series_order == 3) {
-                if (pl) bp_gather_kernel<I, MO, kPrecSeries, true, 3><<<grid, kThreads, 0, s>>>(k);
-                else bp_gather_kernel<I, MO, kPrecSeries, false, 3><<<grid, kThreads, 0, s>>>(k);
+                if (pl) {
+                    if constexpr (I == kInterpLinear && MO) {
+                        if (k.bp_pairs) {
+                            bp_gather_kernel<I, MO, kPrecSeries, true, 3, true>
+                                <<<grid, kThreads, 0, s>>>(k);
+                            break;
+                        }
+                    }
+                    bp_gather_kernel<I, MO, kPrecSeries, true, 3><<<grid, kThreads, 0, s>>>(k);
+                } else {
+                    bp_gather_kernel<I, MO, kPrecSeries, false, 3><<<grid, kThreads, 0, s>>>(k);
+                }
             } else {
                 if (pl) bp_gather_kernel<I, MO, kPrecSeries, true><<<grid, kThreads, 0, s>>>(k);
                 else bp_gather_kernel<I, MO, kPrecSeries, false><<<grid, kThreads, 0, s>>>(k);

@@ -61,6 +61,7 @@ struct KernelParams {
     double2 *bp_partial;      // [bp_splits][M] partial images (tile order) when bp_splits > 1
     const double *bp_cu;      // [num_tiles][Pv] series `cu` moved into [shift_lo, shift_lo + nw)
     const SeriesGather *series_g;  // [num_tiles][Pv] gather records (monostatic planar, order 3) or NULL
+    const float4 *bp_pairs;   // [P][ld] demodulated adjoint row pairs (see pair_rows_kernel) or NULL
     int series_order;         // series order of the linear gather / scatter (3 or 5)
     // variant
     int interp, precision;
@@ -108,6 +109,10 @@ hipError_t launch_truncate64(const double2 *G, const float2 *W, const float2 *wf
                              int P, int centred, float2 *Y, hipStream_t s);
 hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s);
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s);
+// demodulated pairs of the adjoint profile rows over each valid row's span and
+// its tail copies: pairs[i] = (h[i], h[i+1] - h[i]) e^{-2 pi i rho (i + 1/2)}
+hipError_t launch_pair_rows(const float2 *prof, const int2 *spans, const int *rows, int Pv,
+                            int nw, int ld, double rho, float4 *pairs, hipStream_t s);
 hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 *dst,
                               hipStream_t s);
 hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const PulseConst *pc,

@@ -135,6 +135,7 @@ struct sarhip_plan_s {
                                                // (threshold on kappa, its square)
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     SeriesGather *d_series_g = nullptr;  // [num_tiles][Pv] gather records (see SeriesGather)
+    float4 *d_bp_pairs = nullptr;  // [P][ld] demodulated adjoint row pairs (gather pair path)
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
@@ -178,6 +179,7 @@ struct sarhip_plan_s {
         dev_free(d_series);
         dev_free(d_bp_cu);
         dev_free(d_series_g);
+        dev_free(d_bp_pairs);
         dev_free(d_srec);
         dev_free(d_fft_tables);
         dev_free(d_fft_tables64);
@@ -225,6 +227,7 @@ struct sarhip_plan_s {
         k.series = d_series;
         k.bp_cu = d_bp_cu;
         k.series_g = d_series_g;
+        k.bp_pairs = d_bp_pairs;
         k.series_order = series_order;
         k.prof = d_prof;
         k.nw = (int)nw;
@@ -1020,6 +1023,14 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         }
     }
     if (pl->fused_U > 0) {
+        // The gather's pair path (pair_rows_kernel): its pixel phase
+        // rho f + phi' is formed in FP32, so rho must be an FP32 number (to
+        // 2^-34 revolutions per bin) and at most 2.5 (|phase| < 1.75).
+        // SARHIP_BP_PAIRS=0 turns it off.
+        const double rho = pl->kc_rev / pl->by_dx;
+        if (pl->d_series_g && pl->Pv > 0 && std::fabs(rho) <= 2.5 &&
+            std::fabs(rho - (double)(float)rho) <= 0x1p-34 && env_int("SARHIP_BP_PAIRS", 1))
+            HIP_TRY(hipMalloc((void **)&pl->d_bp_pairs, sizeof(float4) * P * pl->ld));
         HIP_TRY(hipDeviceSynchronize());
         owner.p = nullptr;
         *out = pl;
@@ -1230,6 +1241,12 @@ int sarhip_plan_forward_fp64_rows(sarhip_plan_t plan, double *kappa, int *rows) 
     return SARHIP_OK;
 }
 
+int sarhip_plan_gather_pairs(sarhip_plan_t plan, int *enabled) {
+    if (!plan || !enabled) return fail(SARHIP_EINVAL, "NULL argument");
+    *enabled = plan->d_bp_pairs ? 1 : 0;
+    return SARHIP_OK;
+}
+
 int sarhip_plan_set_timing(sarhip_plan_t plan, int enable) {
     if (!plan) return fail(SARHIP_EINVAL, "NULL plan");
     plan->timing = enable != 0;
@@ -1355,6 +1372,9 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
                               pl->d_wfac_adj, (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
                               order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof,
                               pl->fft_sparse, s));
+        if (pl->d_bp_pairs)
+            HIP_TRY(launch_pair_rows(pl->d_prof, pl->d_aspan, pl->d_rows, pl->Pv, (int)pl->nw,
+                                     (int)pl->ld, pl->kc_rev / pl->by_dx, pl->d_bp_pairs, s));
     } else {
         HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
                                   (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,

@@ -145,6 +145,10 @@ void plans(const Problem &p, int interp, int precision) {
     CHECK(sarhip_plan_forward_fp64_rows(pl, &kappa, &redone) == SARHIP_OK && kappa >= 0.0 &&
               redone >= 0 && redone <= (int)p.P && (precision != 0 || kappa == 0.0),
           "forward_fp64_rows");
+    int pairs = -1;
+    CHECK(sarhip_plan_gather_pairs(pl, &pairs) == SARHIP_OK && (pairs == 0 || pairs == 1) &&
+              (pairs == 0 || (precision == 1 && interp == 2)),
+          "gather_pairs");
     CHECK(sarhip_plan_set_timing(pl, 1) == SARHIP_OK, "set_timing");
     std::vector<float> Yt, Xt;
     run_plan(pl, p, Yt, Xt);

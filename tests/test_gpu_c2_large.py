@@ -4,8 +4,9 @@ series ranges, scene of 500 point scatterers plus a 64 x 64 clutter patch,
 seed 1002), with the plan the benchmark builds.
 
 The plan's layout is the one the benchmark times (63-bin forward windows of
-2-tile regions, 64-pulse groups, third-order gather records, fused spectral
-stages at U = 8); the test asserts it, so a layout change shows up here
+2-tile regions, 64-pulse groups, third-order gather records read with
+demodulated pair rows, fused spectral stages at U = 8); the test asserts it,
+so a layout change shows up here
 before it reaches the benchmark unchecked.
 
 The FP64 oracle evaluates exact subsets of the outputs: every output pixel of
@@ -27,7 +28,7 @@ torch = pytest.importorskip("torch")
 
 BENCH_LAYOUT = {"precision": "series", "monostatic": True, "valid_pulses": 2048,
                 "fwd_window_bins": 63, "fwd_pulses_per_group": 64, "fwd_region_tiles": 2,
-                "bp_series_order": 3, "fused_fft_ratio": 8}
+                "bp_series_order": 3, "fused_fft_ratio": 8, "bp_pairs": True}
 
 
 @pytest.fixture(scope="module")
