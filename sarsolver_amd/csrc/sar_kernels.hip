@@ -687,7 +687,7 @@ __global__ __launch_bounds__(256) void bp_reduce_kernel(const double2 *__restric
 // row is poisoned instead.
 //
 // Windows are written whole (zeros included) to `partial` as their integer
-// sums, with their base bins and max |component| (in their units), then
+// sums, with liveness flags (the bases are fixed per plan), then
 // merged per pulse by fp_reduce_kernel in FP64 in a fixed order: no global
 // atomics on the main path, and a result that does not depend on scheduling.
 // ---------------------------------------------------------------------------
@@ -982,7 +982,9 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     // linear series / hilo: u is evaluated relative to the window base
     constexpr bool kRel = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
     constexpr bool kCompact = kRel && PLANAR && NT == 512;
-    const double wb = window_base<MONO>(k, region, p);
+    // window base of (region, pulse): fixed by the geometry, computed once per
+    // plan (window_base_kernel)
+    const double wb = k.wbases[(size_t)j * k.num_regions + region];
     const int wst = k.win_stride;
     for (int e = tid; e < G * wst; e += NT) wsum[e] = 0ull;
     __syncthreads();
@@ -1186,14 +1188,13 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             atomicAdd(&row[0].x, __builtin_nanf(""));
     }
     __syncthreads();
-    // Partials and per-window maxima: wave w writes the integer windows of
-    // pulses w, w + NT/64, ... (coalesced rows) and reduces their maxima in
-    // registers (|sum| < 2^31, so the magnitudes do not overflow); the maximum
-    // is kept in the window's own units, 0 for a window that is all zero.
+    // Partials and liveness: wave w writes the integer windows of pulses w,
+    // w + NT/64, ... (coalesced rows) and whether any of their sums is
+    // non-zero (one ballot; the merge skips windows that are all zero).
     const int NR = k.num_regions;
     const int lane = tid & 63, wave = tid >> 6;
     for (int qq = wave; qq < np; qq += NT / 64) {
-        int mx = 0;
+        bool nz = false;
         int2 *dst = k.partial + ((size_t)(j0 + qq) * NR + region) * win;
         for (int l = lane; l < win; l += 64) {
             const unsigned long long w = wsum[qq * wst + l];
@@ -1201,12 +1202,22 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                                               (int)(unsigned)(w >> 32))
                                   : make_int2(0, 0);
             dst[l] = v;
-            mx = max(mx, max(abs(v.x), abs(v.y)));
+            nz = nz || (v.x | v.y) != 0;
         }
-        for (int off = 32; off > 0; off >>= 1) mx = max(mx, __shfl_xor(mx, off, 64));
-        if (lane == 0) k.wmax[(size_t)(j0 + qq) * NR + region] = (float)mx;
+        const bool live = __ballot(nz) != 0ull;
+        if (lane == 0) k.wlive[(size_t)(j0 + qq) * NR + region] = live ? 1.f : 0.f;
     }
-    if (tid < np) k.wbases[(size_t)(j0 + tid) * NR + region] = wb;
+}
+
+// Once per plan: the window base of every (valid pulse j, forward region),
+// wbases[j][region] = window_base(region, rows[j]).
+template <bool MONO>
+__global__ __launch_bounds__(256) void window_base_kernel(const KernelParams k) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t NR = (size_t)k.num_regions;
+    if (i >= (size_t)k.Pv * NR) return;
+    const int j = (int)(i / NR), region = (int)(i % NR);
+    k.wbases[i] = window_base<MONO>(k, region, k.rows[j]);
 }
 
 // One workgroup per valid pulse: merge the pulse's window partials into its
@@ -1245,7 +1256,7 @@ __global__ __launch_bounds__(kRedThreads) void fp_reduce_kernel(const KernelPara
     const int NR = k.num_regions, win = k.win;
     int *sr = (int *)(sc + NR);
     const double *b = k.wbases + (size_t)j * NR;
-    const float *wm = k.wmax + (size_t)j * NR;
+    const float *wm = k.wlive + (size_t)j * NR;
     const int2 *part = k.partial + (size_t)j * NR * win;
     const int tid = (int)threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -1653,6 +1664,15 @@ hipError_t launch_region_l1(const KernelParams &k, hipStream_t s) {
                                                         k.region_tiles * kTilePixels, k.win > 0,
                                                         k.region_l1, k.region_cmax, k.pixrec,
                                                         const_cast<ScatRec *>(k.srec));
+    return hipGetLastError();
+}
+
+hipError_t launch_window_bases(const KernelParams &k, hipStream_t s) {
+    const size_t n = (size_t)k.Pv * (size_t)k.num_regions;
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    if (k.mono) window_base_kernel<true><<<blocks, 256, 0, s>>>(k);
+    else window_base_kernel<false><<<blocks, 256, 0, s>>>(k);
     return hipGetLastError();
 }
 

@@ -48,8 +48,8 @@ struct KernelParams {
     int win;                  // LDS window length per pulse (0 = no windows)
     int win_stride;           // odd per-pulse stride of the LDS windows (fwd_window_stride)
     int2 *partial;            // [Pv][num_regions][win] integer window sums
-    double *wbases;           // [Pv][num_regions]
-    float *wmax;              // [Pv][num_regions] max |component| of each window (its units)
+    double *wbases;           // [Pv][num_regions] window bases (once per plan, window_base_kernel)
+    float *wlive;             // [Pv][num_regions] 1 for a window with a non-zero sum, else 0
     int2 *fspan;              // [P] covered bins of each forward row: {first bin, count}; count
                               // -1: the row is poisoned (a contribution could not be windowed)
     int *fpoison;             // [P] set by the scatter, read and cleared by the merge
@@ -169,6 +169,8 @@ hipError_t launch_spectral_filter(const float2 *in, const float2 *din, int B, in
                                   const float *f, float2 *work, float2 *out, const float2 *dout,
                                   hipStream_t s);
 hipError_t launch_fp_bin(const KernelParams &k, hipStream_t s);
+// once per plan: k.wbases[j][region] for every valid pulse j and forward region
+hipError_t launch_window_bases(const KernelParams &k, hipStream_t s);
 // dynamic LDS bytes of the merge for the number of forward regions and the window length
 size_t fp_reduce_lds(int num_regions, int win);
 hipError_t launch_fp_reduce(const KernelParams &k, hipStream_t s);

@@ -142,7 +142,7 @@ struct sarhip_plan_s {
     float2 *d_amps_perm = nullptr;
     int2 *d_partial = nullptr;
     double *d_wbases = nullptr;
-    float *d_wmax = nullptr;
+    float *d_wlive = nullptr;
     int2 *d_fspan = nullptr;  // [P] forward row spans written by the merge
     int *d_fpoison = nullptr;  // [P] rows the scatter could not window (cleared by the merge)
     int2 *d_aspan = nullptr;  // [P] adjoint row spans the backprojection reads (fused path)
@@ -192,7 +192,7 @@ struct sarhip_plan_s {
         dev_free(d_amps_perm);
         dev_free(d_partial);
         dev_free(d_wbases);
-        dev_free(d_wmax);
+        dev_free(d_wlive);
         dev_free(d_fspan);
         dev_free(d_fpoison);
         dev_free(d_aspan);
@@ -249,7 +249,7 @@ struct sarhip_plan_s {
         k.win_stride = fwd_window_stride(win);
         k.partial = d_partial;
         k.wbases = d_wbases;
-        k.wmax = d_wmax;
+        k.wlive = d_wlive;
         k.fspan = d_fspan;
         k.fpoison = d_fpoison;
         k.fwd_spans = 0;
@@ -907,7 +907,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     HIP_TRY(hipMalloc((void **)&pl->d_amps_perm, sizeof(float2) * M));
     const size_t nrows = std::max<size_t>(1, (size_t)pl->Pv);
     HIP_TRY(hipMalloc((void **)&pl->d_wbases, sizeof(double) * nrows * pl->num_regions));
-    HIP_TRY(hipMalloc((void **)&pl->d_wmax, sizeof(float) * nrows * pl->num_regions));
+    HIP_TRY(hipMalloc((void **)&pl->d_wlive, sizeof(float) * nrows * pl->num_regions));
+    if (pl->Pv > 0) HIP_TRY(launch_window_bases(pl->params(), nullptr));
     // rows the merge never writes (invalid pulses) keep the empty span
     HIP_TRY(hipMalloc((void **)&pl->d_fspan, sizeof(int2) * std::max<size_t>(P, 1)));
     HIP_TRY(hipMemset(pl->d_fspan, 0, sizeof(int2) * std::max<size_t>(P, 1)));
