@@ -511,7 +511,7 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
     // kHG: one SeriesGather record per (tile, pulse) replaces the series
     // constants and the row shift (Horner form, phase from u).
     constexpr bool kHG = kShift && MONO && PLANAR && PREC == kPrecSeries && SORD == 3;
-    const double rho = k.kc_rev / k.by_dx;
+    const double rho = k.rho;
     const float rho32 = (float)rho;
     int p_next = 0;
     PulseData<MONO> d_next;
@@ -967,7 +967,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     const int t1 = min(k.num_tiles, t0 + R);
 
     const PolyRegs pr = make_poly_regs<SORD>();
-    const double rho = k.kc_rev / k.by_dx;  // revolutions per bin
+    const double rho = k.rho;  // revolutions per bin
     // region bound (region_l1_kernel) -> fixed-point scale
     const float bound = k.region_l1[region];
     // region-uniform flags, made scalar so that they branch on the scalar unit

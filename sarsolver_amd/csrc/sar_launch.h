@@ -31,6 +31,7 @@ struct KernelParams {
     float2 *prof;
     int nw;
     double nwd, inv_nw, by_dx, kc_rev;
+    double rho;  // kc_rev / by_dx: phase revolutions per bin
     // adjoint profile rows: stride ld = nw + pad, bins [nw, ld) repeat bins
     // [0, ld - nw) (mod nw); shift_lo bounds |u - u(tile centre)| in bins
     int ld;

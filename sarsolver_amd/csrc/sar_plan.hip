@@ -237,6 +237,7 @@ struct sarhip_plan_s {
         k.inv_nw = inv_nw;
         k.by_dx = by_dx;
         k.kc_rev = kc_rev;
+        k.rho = kc_rev / by_dx;
         k.out = nullptr;
         k.amps_perm = d_amps_perm;
         k.num_regions = num_regions;
