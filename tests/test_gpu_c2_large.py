@@ -15,7 +15,9 @@ corners and centre of the clutter patch, and 2048 random pixels), and every
 output pulse of A depends on no other pulse (32 pulses spread over the
 aperture, for the whole image).  Bar: relative L2 <= 1e-5 for both operators,
 on the scene's phase history (A^H), on the dense complex Gaussian image the
-benchmark's forward is timed on (A), and on the scene itself (A).
+benchmark's forward is timed on (A), and on the scene itself (A); and the
+dot-product test of the whole operator pair (linear interpolation is an exact
+adjoint pair) at 1e-6.
 """
 import numpy as np
 import pytest
@@ -57,6 +59,15 @@ def c2(dev, oracle_lib):
     # the dense image does not cancel (kappa ~0.8): no row needs the FP64 pass
     dense_fp64_rows = pl.forward_fp64_rows()
     Ys = y.cpu().numpy()
+    # dot-product test at the full size (linear interpolation is an exact
+    # adjoint pair): <A x, y> against <x, A^H y> in FP64 for the dense image
+    # and a seeded complex Gaussian phase history
+    yr = synthetic.complex_normal(np.random.default_rng(w.seed + 7), (w.pulses, w.samples))
+    Ayr = pl.adjoint(torch.from_numpy(yr.astype(np.complex64)).to(dev)).cpu().numpy()
+    xd64 = xdense.astype(np.complex64).astype(np.complex128)
+    lhs = np.vdot(yr.astype(np.complex64).astype(np.complex128), Yd.astype(np.complex128))
+    rhs = np.vdot(Ayr.astype(np.complex128), xd64)
+    dot = abs(lhs - rhs) / (np.linalg.norm(Yd) * np.linalg.norm(yr.astype(np.complex64)))
     del pl
     torch.cuda.empty_cache()
     # pixel subset: the 500 scatterers (the workload's first draw), the clutter
@@ -93,7 +104,7 @@ def c2(dev, oracle_lib):
                                interp="linear", num_threads=8)
     print("oracle done", flush=True)
     return dict(w=w, lay=lay, X=X[pix], X_or=X_or, Yd=Yd[rows], Yd_or=Yd_or,
-                Ys=Ys[rows], Ys_or=Ys_or, dense_fp64_rows=dense_fp64_rows)
+                Ys=Ys[rows], Ys_or=Ys_or, dense_fp64_rows=dense_fp64_rows, dot=dot)
 
 
 def test_c2_plan_layout_is_the_benchmarked_one(c2):
@@ -125,3 +136,9 @@ def test_c2_forward_scene_pulse_rows_against_oracle(c2):
     e = rel_l2(c2["Ys"], c2["Ys_or"])
     print(f"c2 A (scene) on 32 pulses: rel L2 {e:.2e}")
     assert e <= 1e-5
+
+
+def test_c2_dot_product_at_full_size(c2):
+    """|<A x, y> - <x, A^H y>| / (||A x|| ||y||) <= 1e-6 on the benchmarked plan."""
+    print(f"c2 dot-product test: {c2['dot']:.2e}")
+    assert c2["dot"] <= 1e-6
