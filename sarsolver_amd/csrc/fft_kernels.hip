@@ -236,6 +236,14 @@ __device__ __forceinline__ void fft_lds(C *sh, const Twiddles<C> &tw, int gt) {
 
 __device__ __forceinline__ int signed_freq(const int k, const int N) { return k < N / 2 ? k : k - N; }
 
+// Residue of s r modulo N_w for a signed band frequency s in [-N/2, N/2) and a
+// phase r < U: s r lies in (-N_w/2, N_w/2), so it is s r or s r + N_w (no
+// division; |s| < 2^13 and r < 2^11 fit the 24-bit multiply).
+__device__ __forceinline__ int band_phase(const int s, const int r, const int nw) {
+    const int t = __mul24(s, r);
+    return t < 0 ? t + nw : t;
+}
+
 // Q phases per batch share the LDS (Q buffers): two 256-thread groups (one
 // for Q = 1) take Q / 2 phases each in turn, then every m's Q consecutive
 // phases are one contiguous store / load of 8 Q bytes of the profile row.
@@ -327,20 +335,26 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
             float2 *sh = lds + buf * SZ;
             if constexpr (N >= 16 * T) {
                 // the first radix-16 pass straight from registers: butterfly
-                // j = gt + T b takes k = j + (N / 16) r, held in B[b + PB r]
+                // j = gt + T b takes k = j + (N / 16) r, held in B[b + PB r].
+                // Its signed frequency is j + (N / 16) r' (r' = r, or r - 16
+                // past N / 2), so the phase factor e^{2 pi i s_k rp / N_w}
+                // splits into e^{2 pi i j rp / N_w} (j rp < N_w / 16), one per
+                // butterfly, and e^{2 pi i (N / 16) r' rp / N_w}, the same for
+                // the whole wave and, N / 16 being a multiple of 256, one
+                // entry of the coarse table.
                 constexpr int PB = N / (16 * T);
+                const int rpu = __builtin_amdgcn_readfirstlane(rp);
 #pragma unroll
                 for (int b = 0; b < PB; ++b) {
+                    const int j = gt + T * b;
+                    const float2 c1 = conjf2(tw.w(j * rpu));
                     float2 v[16];
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
-                        const int k = gt + T * (b + PB * r);
-                        int t = (signed_freq(k, N) * rp) % nw;
-                        t = t < 0 ? t + nw : t;
-                        v[r] = cmulf2(B[b + PB * r], conjf2(tw.w(t)));
+                        const int t2 = band_phase((r < 8 ? r : r - 16) * (N / 16), rpu, nw);
+                        v[r] = cmulf2(B[b + PB * r], cmulf2(c1, conjf2(tw.whi[t2 >> 8])));
                     }
                     dft<16, true>(v);
-                    const int j = gt + T * b;
 #pragma unroll
                     for (int r = 0; r < 16; ++r) sh[phys(16 * j + r)] = v[r];
                 }
@@ -351,8 +365,7 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
                 for (int i = 0; i < NI; ++i) {
                     const int k = gt + T * i;
                     if (k < N) {
-                        int t = (signed_freq(k, N) * rp) % nw;
-                        t = t < 0 ? t + nw : t;
+                        const int t = band_phase(signed_freq(k, N), rp, nw);
                         sh[phys(k)] = cmulf2(B[i], conjf2(tw.w(t)));
                     }
                 }
@@ -507,13 +520,28 @@ __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
                     const int k = tid + Bt::NT * q;
                     if (k >= N) break;
                     if (q == 0 || q == PER / 2) {
-                        int t = (signed_freq(k, N) * r) % nw;
-                        t = t < 0 ? t + nw : t;
-                        w = tw.w(t);
+                        w = tw.w(band_phase(signed_freq(k, N), r, nw));
                     } else {
                         w = cmul(w, step);
                     }
                     acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(k)], w));
+                }
+            }
+        } else if constexpr (N >= 2 * Bt::NT) {
+            // k = tid + NT q has the signed frequency tid + NT q' (q' = q, or
+            // q - PER past N / 2), so e^{-2 pi i s_k r / N_w} splits into
+            // e^{-2 pi i tid r / N_w} (tid r < N_w / 2), one per thread and
+            // phase, and e^{-2 pi i NT q' r / N_w}, the same for the whole
+            // wave and, NT being a multiple of 256, one entry of the coarse table.
+#pragma unroll
+            for (int qq = 0; qq < Q; ++qq) {
+                const int r = __builtin_amdgcn_readfirstlane(r0 + qq);
+                const C c1 = tw.w(tid * r);
+#pragma unroll
+                for (int q = 0; q < PER; ++q) {
+                    const int t2 = band_phase((q < PER / 2 ? q : q - PER) * Bt::NT, r, nw);
+                    const C w = cmul(c1, tw.whi[t2 >> 8]);
+                    acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(tid + Bt::NT * q)], w));
                 }
             }
         } else {
@@ -523,11 +551,8 @@ __global__ __launch_bounds__((DownBatch<N, Q, C>::NT)) void fft_down_kernel(
                 if (k >= N) break;
                 const int s = signed_freq(k, N);
 #pragma unroll
-                for (int qq = 0; qq < Q; ++qq) {
-                    int t = (s * (r0 + qq)) % nw;
-                    t = t < 0 ? t + nw : t;
-                    acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(k)], tw.w(t)));
-                }
+                for (int qq = 0; qq < Q; ++qq)
+                    acc[q] = cadd(acc[q], cmul(lds[qq * SZ + phys(k)], tw.w(band_phase(s, r0 + qq, nw))));
             }
         }
         __syncthreads();
