@@ -38,6 +38,10 @@ constexpr int kBufferWord3 = 0x00020000;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 __device__ i32x4 llvm_struct_buffer_load_b128(i32x4 rsrc, int vindex, int voffset, int soffset,
                                               int aux) __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+// The same, 8 bytes (the first half of a 16-byte pair entry).
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+__device__ i32x2 llvm_struct_buffer_load_b64(i32x4 rsrc, int vindex, int voffset, int soffset,
+                                             int aux) __asm("llvm.amdgcn.struct.buffer.load.v2i32");
 // Resource over one profile row of `ld` complex64 bins as records of 8 bytes,
 // num_records = ld - 1: a load at bin n reads bins n and n + 1 for n <= ld - 2
 // and zeros beyond, so it never reads past the row.  The bin index goes to the
@@ -61,6 +65,50 @@ __device__ __forceinline__ i32x4 pair_rsrc(const float4 *row, const int ld) {
     r.z = ld - 1;
     r.w = kBufferWord3;
     return r;
+}
+
+// Read-only elements at wave-uniform indices, through the constant address
+// space so that the compiler uses scalar loads (for global pointers it must
+// first prove that no store of the kernel reaches the memory, a proof it
+// abandons in the larger loops).
+typedef const __attribute__((address_space(4))) double *const_f64_ptr;
+typedef const __attribute__((address_space(4))) int *const_i32_ptr;
+static_assert(offsetof(SeriesGather, phi) == 48 && offsetof(SeriesGather, phi_pair) == 56 &&
+                  sizeof(SeriesGather) == 64,
+              "SeriesGather field order");
+__device__ __forceinline__ SeriesGather load_uniform(const SeriesGather *p, size_t i) {
+    const const_f64_ptr d = (const_f64_ptr)(p + i);
+    SeriesGather g;
+    g.bx = d[0];
+    g.by = d[1];
+    g.k1 = d[2];
+    g.k2 = d[3];
+    g.k3 = d[4];
+    g.cu = d[5];
+    g.phi = d[6];
+    const long long w = __double_as_longlong(d[7]);
+    g.phi_pair = __uint_as_float((unsigned)w);
+    g.pad = __uint_as_float((unsigned)(w >> 32));
+    return g;
+}
+__device__ __forceinline__ int load_uniform(const int *p, size_t i) {
+    return ((const_i32_ptr)(p + i))[0];
+}
+static_assert(offsetof(PulseConst, rx) == 24 && offsetof(PulseConst, r_srp) == 48 &&
+                  sizeof(PulseConst) == 64,
+              "PulseConst field order");
+__device__ __forceinline__ PulseConst load_uniform(const PulseConst *p, size_t i) {
+    const const_f64_ptr d = (const_f64_ptr)(p + i);
+    PulseConst c;
+    c.tx[0] = d[0];
+    c.tx[1] = d[1];
+    c.tx[2] = d[2];
+    c.rx[0] = d[3];
+    c.rx[1] = d[4];
+    c.rx[2] = d[5];
+    c.r_srp = d[6];
+    c.valid = d[7];
+    return c;
 }
 
 // Two consecutive profile samples (8-byte aligned) read as one 16-byte load.
@@ -509,18 +557,20 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
     // bins are unchanged modulo nw and the padded row tail makes bins
     // n + 1 <= ld - 1 valid without a wrap.
     // kHG: one SeriesGather record per (tile, pulse) replaces the series
-    // constants and the row shift (Horner form, phase from u).
-    constexpr bool kHG = kShift && MONO && PLANAR && PREC == kPrecSeries && SORD == 3;
+    // constants and the row shift (Horner form, phase from u); the nearest
+    // rules read them on the pair path only, with the rule's offset in cu.
+    constexpr bool kHG = (kShift || PAIRS) && MONO && PLANAR && PREC == kPrecSeries && SORD == 3;
     const double rho = k.rho;
     const float rho32 = (float)rho;
     int p_next = 0;
     PulseData<MONO> d_next;
     SeriesGather g_next;
     if (j_begin < j_end) {
-        p_next = k.rows[j_begin];
         if constexpr (kHG) {
-            g_next = k.series_g[(size_t)tile * k.Pv + j_begin];
+            p_next = load_uniform(k.rows, (size_t)j_begin);
+            g_next = load_uniform(k.series_g, (size_t)tile * k.Pv + j_begin);
         } else {
+            p_next = k.rows[j_begin];
             load_pulse<PREC, MONO>(k, tile, j_begin, p_next, d_next);
             if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j_begin];
         }
@@ -530,10 +580,11 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
         PulseData<MONO> d = d_next;
         const SeriesGather g = g_next;
         if (j + 1 < j_end) {
-            p_next = k.rows[j + 1];
             if constexpr (kHG) {
-                g_next = k.series_g[(size_t)tile * k.Pv + j + 1];
+                p_next = load_uniform(k.rows, (size_t)j + 1);
+                g_next = load_uniform(k.series_g, (size_t)tile * k.Pv + j + 1);
             } else {
+                p_next = k.rows[j + 1];
                 load_pulse<PREC, MONO>(k, tile, j + 1, p_next, d_next);
                 if constexpr (kShift) d_next.s.cu = k.bp_cu[(size_t)tile * k.Pv + j + 1];
             }
@@ -542,7 +593,71 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
         // 16-byte loads of bins (n, n + 1) through a structured buffer over the
         // padded row: indices beyond it read zeros instead of faulting.
         const i32x4 rsrc = row_pair_rsrc(row, k.ld);
-        if constexpr (PAIRS) {
+        if constexpr (PAIRS && INTERP != kInterpLinear) {
+            // nearest rules on the demodulated pair rows: the record's cu
+            // carries the rule's offset o (1 for round(1/2 + u), 1/2 for
+            // round(u)), so n = floor(u + o) and f = u + o - n; one 8-byte
+            // load of A_n = h[n] e^{-2 pi i rho (n + 1/2)}, whose pixel phase is
+            // rho f + phi' as on the linear pair path.  Within 2e-7 bin of a
+            // bin edge the reference's rule decides on the exact range
+            // (CPP:346-347, CPP:289), a rare lane-divergent branch.
+            static_assert(kHG, "the pair path reads SeriesGather records");
+            constexpr bool REF = INTERP == kInterpNearestRef;
+            const i32x4 psrc = pair_rsrc(k.bp_pairs + (size_t)p * k.ld, k.ld);
+            double u[kPixPerThread];
+            int n0[kPixPerThread];
+            float f[kPixPerThread];
+            bool tie = false;
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, q[m].d2));
+                u[m] = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
+                n0[m] = (int)u[m];
+                f[m] = (float)__builtin_amdgcn_fract(u[m]);
+                tie = tie || fabsf(f[m] - 0.5f) > 0.5f - 2.0f * (float)kTieMargin;
+            }
+            if (__builtin_amdgcn_read_exec() & __ballot(tie)) {
+#pragma unroll
+                for (int m = 0; m < kPixPerThread; ++m) {
+                    if (fabsf(f[m] - 0.5f) > 0.5f - 2.0f * (float)kTieMargin) {
+                        // the exact range of the pixel (as load_pix places it)
+                        const int i = tile * kTilePixels + m * kThreads + tid;
+                        int ii = i < k.M ? i : k.M - 1;
+                        // formed here, not kept in registers across the pulse loop
+                        asm volatile("" : "+v"(ii));
+                        double x = k.px[ii], y = k.py[ii], z = k.pz[ii];
+                        if (!q[m].ok) {
+                            x = X0.x;
+                            y = X0.y;
+                            z = X0.z;
+                        }
+                        const double ue = ref_dr<MONO>(load_uniform(k.pc, (size_t)p), x, y, z) * k.by_dx;
+                        const double ne = round(REF ? 0.5 + ue : ue);
+                        // u - o - ue is the row shift, a multiple of nw, to 1e-9 bin
+                        const int ns = (int)(ne + rint(u[m] - (REF ? 1.0 : 0.5) - ue));
+                        f[m] -= (float)(ns - n0[m]);
+                        n0[m] = ns;
+                    }
+                    // one pixel at a time (keeps the registers of the rare path few)
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+            i32x2 h[kPixPerThread];
+            float sn[kPixPerThread], cs[kPixPerThread];
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                h[m] = llvm_struct_buffer_load_b64(psrc, n0[m], 0, 0, 0);
+                const float th = fmaf(rho32, f[m], g.phi_pair);
+                sn[m] = __builtin_amdgcn_sinf(th);
+                cs[m] = __builtin_amdgcn_cosf(th);
+            }
+#pragma unroll
+            for (int m = 0; m < kPixPerThread; ++m) {
+                const float hr = __int_as_float(h[m].x), hi = __int_as_float(h[m].y);
+                ar[m] = fmaf(hr, cs[m], fmaf(hi, sn[m], ar[m]));
+                ai[m] = fmaf(hi, cs[m], fmaf(-hr, sn[m], ai[m]));
+            }
+        } else if constexpr (PAIRS) {
             // demodulated pair rows (pair_rows_kernel): one 16-byte load of
             // (A_n, D_n), the phase rho f + phi' in FP32
             static_assert(kHG, "the pair path reads SeriesGather records");
@@ -727,9 +842,6 @@ __device__ __forceinline__ float fix_undo(const FixScale &f, const float v) {
 }
 // Whether a bound admits fixed-point windows: finite with 1.001 b finite.
 __device__ __forceinline__ bool fix_bound_ok(const float b) { return isfinite(b * 1.001f); }
-// Read-only element at a wave-uniform index, through the constant address
-// space so that the compiler may use scalar loads.
-typedef const __attribute__((address_space(4))) double *const_f64_ptr;
 static_assert(offsetof(PixRec, dx) == 0 && offsetof(PixRec, d2) == 24 &&
                   offsetof(PixRec, x) == 32 && offsetof(PixRec, ok) == 56,
               "PixRec field order");
@@ -979,8 +1091,12 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
     FixScale fs{1.f, 1.f, 0};
     if (scaled) fs = fix_scale(bound);
     auto unscale = [&](const float v) { return scaled ? fix_undo(fs, v) : v; };
-    // linear series / hilo: u is evaluated relative to the window base
-    constexpr bool kRel = INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo);
+    // linear series / hilo, and the nearest rules with series ranges: u is
+    // evaluated relative to the window base (for the nearest rules plus 1/2,
+    // so that truncation gives round(u) away from ties)
+    constexpr bool kNear = INTERP != kInterpLinear && PREC == kPrecSeries;
+    constexpr bool kRel =
+        (INTERP == kInterpLinear && (PREC == kPrecSeries || PREC == kPrecHilo)) || kNear;
     constexpr bool kCompact = kRel && PLANAR && NT == 512;
     // window base of (region, pulse): fixed by the geometry, computed once per
     // plan (window_base_kernel)
@@ -998,7 +1114,10 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
             for (int tile = t0; tile < t1; ++tile) {
                 PulseData<MONO> d;
                 load_pulse<PREC, MONO>(k, tile, j, p, d);
-                if constexpr (kRel) d.s.cu -= wb;
+                if constexpr (kNear)
+                    d.s.cu = (d.s.cu - wb) + 0.5;
+                else if constexpr (kRel)
+                    d.s.cu -= wb;
                 SeriesHorner<MONO, SORD> hc;
                 if constexpr (PREC == kPrecSeries) hc = make_horner<MONO, SORD>(d, rho);
                 const int tb = tile * kTilePixels;
@@ -1025,19 +1144,37 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         a_next = load_uniform((const float2 *)k.amps_perm, (size_t)(tb + stream));
                     }
                 }
-                // one pixel's contribution to this lane's pulse window
-                auto contribute = [&](const Pix &px, const float2 a) {
+                // one pixel's contribution to this lane's pulse window (ri:
+                // the pixel's record index, for the exact range on kCompact)
+                auto contribute = [&](const Pix &px, const float2 a, const int ri) {
                     double u, th;
                     if constexpr (PREC == kPrecSeries)
                         eval_range_horner<MONO, PLANAR, SORD>(d, hc, px, rho, u, th);
                     else
                         eval_range<PREC, MONO, PLANAR, SORD>(k, d, px, u, th, pr);
                     float sn, cs;
-                    double fl;
+                    double fl = 0.0;
+                    int ln = 0;  // kNear: the window bin
                     if constexpr (INTERP == kInterpLinear && kRel) {
                         // u is taken from the window base, at least 2 bins below
                         // every pixel of the region: truncation is floor
                         fl = (double)(int)u;
+                    } else if constexpr (kNear) {
+                        // u - 1/2 from the window base: truncation is the
+                        // forward rule round(u) (CPP:289) unless u is within
+                        // 1e-7 bin of a tie, where the rule decides on the
+                        // exact range (rare, lane-divergent)
+                        ln = (int)u;
+                        if (fabs(__builtin_amdgcn_fract(u) - 0.5) > 0.5 - kTieMargin) {
+                            double x = px.x, y = px.y, z = px.z;
+                            if constexpr (kCompact) {
+                                const PixRec r = load_uniform(k.pixrec, (size_t)ri);
+                                x = r.x;
+                                y = r.y;
+                                z = r.z;
+                            }
+                            ln = (int)(round(ref_dr<MONO>(k.pc[p], x, y, z) * k.by_dx) - wb);
+                        }
                     } else if constexpr (INTERP == kInterpLinear) {
                         fl = floor(u);
                     } else {
@@ -1052,7 +1189,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                     // contribution a e^{i theta} (times `scale` when pre-scaled)
                     const float cr = fmaf(cs, a.x, -(sn * a.y));
                     const float ci = fmaf(cs, a.y, sn * a.x);
-                    const int l = kRel ? (int)fl : (int)(fl - wb);
+                    const int l = kNear ? ln : kRel ? (int)fl : (int)(fl - wb);
                     if constexpr (kFixed) {
                         // Every lane adds into its window without a branch: a lane
                         // whose bin falls outside the window adds at the spare
@@ -1092,7 +1229,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             atomicAdd(&row[n1].x, unscale(fr * cr));
                             atomicAdd(&row[n1].y, unscale(fr * ci));
                         } else {
-                            const int nb = bin_index(k, fl, 1);
+                            const int nb = bin_index(k, kNear ? (double)ln + wb : fl, 1);
                             atomicAdd(&row[nb].x, unscale(cr));
                             atomicAdd(&row[nb].y, unscale(ci));
                         }
@@ -1122,7 +1259,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                             px.dx = c[e].dx;
                             px.dy = c[e].dy;
                             px.d2 = c[e].d2;
-                            contribute(px, c[e].a);
+                            contribute(px, c[e].a, tb + i + e);
                         }
                     };
                     // The next group is requested after the wait for this one:
@@ -1169,7 +1306,7 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         px.dz = r.dz;
                         px.d2 = r.d2;
                         px.ok = true;
-                        contribute(px, a);
+                        contribute(px, a, 0);
                     }
                 }
             }
@@ -1528,6 +1665,7 @@ hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, c
 __global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__restrict__ series,
                                                             size_t n, double shift_lo,
                                                             double inv_nw, double nwd, double rho,
+                                                            double off,
                                                             SeriesGather *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
@@ -1541,6 +1679,9 @@ __global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__
     g.k3 = s.ku * 0.0625 * (c2 * s.c);
     g.pad = 0.f;
     g.cu = fma(-floor((s.cu - shift_lo) * inv_nw), nwd, s.cu);  // as bp_shift_kernel
+    // the nearest rules' offset (floor(u + off) is their bin away from ties);
+    // phi then refers to u + off
+    g.cu += off;
     g.phi = fma(-rho, g.cu, s.ct);
     // the pair path's phase constant: phi reduced to [-1/2, 1/2], less rho / 2
     // (the pair rows carry e^{-2 pi i rho (n + 1/2)}), so that the pixel phase
@@ -1550,9 +1691,10 @@ __global__ __launch_bounds__(256) void series_gather_kernel(const SeriesMono *__
 }
 
 hipError_t launch_series_gather(const void *series, size_t n, double shift_lo, double inv_nw,
-                                double nwd, double rho, SeriesGather *out, hipStream_t s) {
+                                double nwd, double rho, double off, SeriesGather *out,
+                                hipStream_t s) {
     series_gather_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(
-        (const SeriesMono *)series, n, shift_lo, inv_nw, nwd, rho, out);
+        (const SeriesMono *)series, n, shift_lo, inv_nw, nwd, rho, off, out);
     return hipGetLastError();
 }
 
@@ -1586,7 +1728,15 @@ static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const bool pl = I == kInterpLinear && k.planar;
     switch (k.precision) {
         case kPrecSeries:
-            if (I == kInterpLinear && k.series_order == 3) {
+            if constexpr (I != kInterpLinear && MO) {
+                // nearest rules: the pair path when the plan built pair rows
+                // (monostatic planar tiles, third-order series)
+                if (k.bp_pairs && k.planar && k.series_order == 3) {
+                    bp_gather_kernel<I, MO, kPrecSeries, true, 3, true><<<grid, kThreads, 0, s>>>(k);
+                    break;
+                }
+            }
+            if (k.series_order == 3) {
                 if (pl) {
                     if constexpr (I == kInterpLinear && MO) {
                         if (k.bp_pairs) {
@@ -1641,7 +1791,7 @@ static void fp_dispatch_prec(const KernelParams &k, hipStream_t s) {
     const bool pl = k.planar;
     switch (k.precision) {
         case kPrecSeries:
-            if (I == kInterpLinear && k.series_order == 3) {
+            if (k.series_order == 3) {
                 if (pl) fp_scatter_kernel<I, MO, kPrecSeries, true, NT, 3><<<grid, NT, lds, s>>>(k);
                 else fp_scatter_kernel<I, MO, kPrecSeries, false, NT, 3><<<grid, NT, lds, s>>>(k);
             } else {

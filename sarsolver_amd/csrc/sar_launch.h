@@ -130,9 +130,11 @@ hipError_t launch_region_count(const KernelParams &k, int *cmax, hipStream_t s);
 // backprojection can read (span[p] for p = rows[j]; other entries untouched).
 hipError_t launch_adjoint_spans(const PulseConst *pc, const int *rows, int Pv, const double4 *tiles,
                                 int nt, bool mono, double by_dx, int nw, int2 *span, hipStream_t s);
-// Gather records from the monostatic series constants (cu moved as launch_bp_shift).
+// Gather records from the monostatic series constants (cu moved as launch_bp_shift,
+// plus `off`: 0 for linear interpolation, the nearest rules' offset otherwise).
 hipError_t launch_series_gather(const void *series, size_t n, double shift_lo, double inv_nw,
-                                double nwd, double rho, SeriesGather *out, hipStream_t s);
+                                double nwd, double rho, double off, SeriesGather *out,
+                                hipStream_t s);
 hipError_t launch_bp_shift(const void *series, bool mono, size_t n, double shift_lo, double inv_nw,
                            double nwd, double *out, hipStream_t s);
 // dataset resampling stages (resample_kernels.hip)

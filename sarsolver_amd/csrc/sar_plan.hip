@@ -778,8 +778,9 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
     // geometry does not allow it the series mode falls back to exact FP64.
     pl->requested_precision = precision;
     pl->precision = precision;
-    // The linear kernels of a series plan use the third-order series when its
-    // bound meets the same limits (the nearest modes keep the fifth).
+    // The kernels of a series plan use the third-order series when its bound
+    // meets the same limits.  (The nearest rules stay exact with it: a bin
+    // decision within 1e-7 bin of a tie is taken on the exact range.)
     pl->series_order = 5;
     if (precision == SARHIP_PRECISION_SERIES || precision == SARHIP_PRECISION_HILO) {
         auto ok = [&](double bins) {
@@ -788,8 +789,7 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         pl->series_err_bins = series_error_bound(tiles, tile_half, pc, rows, mono, pl->by_dx, 5);
         if (precision == SARHIP_PRECISION_SERIES && !ok(pl->series_err_bins))
             pl->precision = SARHIP_PRECISION_FP64;
-        if (pl->precision == SARHIP_PRECISION_SERIES && interp == SARHIP_INTERP_LINEAR &&
-            env_int("SARHIP_SERIES_ORDER", 3) == 3) {
+        if (pl->precision == SARHIP_PRECISION_SERIES && env_int("SARHIP_SERIES_ORDER", 3) == 3) {
             const double e3 = series_error_bound(tiles, tile_half, pc, rows, mono, pl->by_dx, 3);
             if (ok(e3)) {
                 pl->series_order = 3;
@@ -926,12 +926,15 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         HIP_TRY(hipMemset(pl->d_region_cmax, 0, sizeof(int) * pl->num_regions));
         HIP_TRY(launch_region_count(pl->params(), pl->d_region_cmax, nullptr));
     }
-    // The planar 64-pulse linear scatter streams 32-byte records that the
+    // The planar 64-pulse scatter of the linear series / hilo modes and of the
+    // nearest rules with series ranges streams 32-byte records that the
     // region-norm kernel writes on every forward call.
     // The image's last record group may run into four spare records, which
     // are zero (zero offset and amplitude: a zero contribution).
-    if (pl->planar && interp == SARHIP_INTERP_LINEAR && pl->fwd_pulses == 64 &&
-        (pl->precision == SARHIP_PRECISION_SERIES || pl->precision == SARHIP_PRECISION_HILO)) {
+    if (pl->planar && pl->fwd_pulses == 64 &&
+        ((interp == SARHIP_INTERP_LINEAR && (pl->precision == SARHIP_PRECISION_SERIES ||
+                                             pl->precision == SARHIP_PRECISION_HILO)) ||
+         (interp != SARHIP_INTERP_LINEAR && pl->precision == SARHIP_PRECISION_SERIES))) {
         HIP_TRY(hipMalloc((void **)&pl->d_srec, sizeof(ScatRec) * (M + 4)));
         HIP_TRY(hipMemset(pl->d_srec + M, 0, sizeof(ScatRec) * 4));
     }
@@ -965,7 +968,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                 pl->series_order == 3) {
                 HIP_TRY(hipMalloc((void **)&pl->d_series_g, sizeof(SeriesGather) * n));
                 HIP_TRY(launch_series_gather(pl->d_series, n, pl->shift_lo, pl->inv_nw, (double)nw,
-                                             pl->kc_rev / pl->by_dx, pl->d_series_g, nullptr));
+                                             pl->kc_rev / pl->by_dx, 0.0, pl->d_series_g,
+                                             nullptr));
             } else {
                 HIP_TRY(hipMalloc((void **)&pl->d_bp_cu, sizeof(double) * n));
                 HIP_TRY(launch_bp_shift(pl->d_series, mono, n, pl->shift_lo, pl->inv_nw,
@@ -1030,8 +1034,21 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
         // 2^-34 revolutions per bin) and at most 2.5 (|phase| < 1.75).
         // SARHIP_BP_PAIRS=0 turns it off.
         const double rho = pl->kc_rev / pl->by_dx;
-        if (pl->d_series_g && pl->Pv > 0 && std::fabs(rho) <= 2.5 &&
-            std::fabs(rho - (double)(float)rho) <= 0x1p-34 && env_int("SARHIP_BP_PAIRS", 1))
+        const bool rho_ok = std::fabs(rho) <= 2.5 &&
+                            std::fabs(rho - (double)(float)rho) <= 0x1p-34 &&
+                            env_int("SARHIP_BP_PAIRS", 1);
+        // the nearest rules' pair path reads gather records whose cu carries
+        // the adjoint rule's offset: 1 for round(1/2 + u) (CPP:346-347), 1/2
+        // for round(u)
+        if (interp != SARHIP_INTERP_LINEAR && rho_ok && mono && pl->planar && pl->d_series &&
+            pl->precision == SARHIP_PRECISION_SERIES && pl->series_order == 3 && pl->Pv > 0) {
+            const size_t n = (size_t)pl->num_tiles * (size_t)pl->Pv;
+            HIP_TRY(hipMalloc((void **)&pl->d_series_g, sizeof(SeriesGather) * n));
+            HIP_TRY(launch_series_gather(pl->d_series, n, pl->shift_lo, pl->inv_nw, (double)nw, rho,
+                                         interp == SARHIP_INTERP_NEAREST_REF ? 1.0 : 0.5,
+                                         pl->d_series_g, nullptr));
+        }
+        if (pl->d_series_g && pl->Pv > 0 && rho_ok)
             HIP_TRY(hipMalloc((void **)&pl->d_bp_pairs, sizeof(float4) * P * pl->ld));
         HIP_TRY(hipDeviceSynchronize());
         owner.p = nullptr;

@@ -104,6 +104,45 @@ def test_exact_ties_follow_the_reference_rule(dev, oracle_lib, precision, interp
     assert rel_l2(Y, Yo) <= 1e-5 and rel_l2(X, Xo) <= 1e-5
 
 
+@pytest.mark.parametrize("interp", ["nearest_ref", "nearest"])
+def test_exact_ties_on_the_pair_path(dev, oracle_lib, interp):
+    """The tie problem moved to 100 km (third-order series bound ~1e-12 bin) with
+    N = 64 samples (fused spectral stages, N_w = 256) and f_c = 1/2, so that
+    rho = f_c / (f_s U) = 1/8 is an FP32 number: the series plan runs the
+    nearest rules on the gather's pair rows and the scatter's window-relative
+    path.  Pixels at multiples of 1/16 m have u = -8x exactly, so every pixel
+    sits on a tie of the forward rule round(u) or of the adjoint rule
+    (round(1/2 + u) or round(u)), and each tie must be decided as the
+    reference decides it."""
+    from sarsolver_amd import SarPlan
+    P, N, U = 4, 64, 4.0
+    tx = np.tile(np.array([[1.0e5, 0.0, 0.0]]), (P, 1))
+    xs = np.arange(-64, 64) / 16.0
+    pos = np.stack([xs, np.zeros_like(xs), np.zeros_like(xs)], 1)
+    rng = np.random.default_rng(29)
+    a = rng.standard_normal(len(xs)) + 1j * rng.standard_normal(len(xs))
+    y = rng.standard_normal((P, N)) + 1j * rng.standard_normal((P, N))
+    W = np.ones(N, complex)
+    w = np.exp(1j * np.arange(P))
+    fc, fs, c = 0.5, 1.0, 1.0
+    pl = SarPlan(tx, tx, np.zeros_like(tx), pos, W, w, fc, fs, c, U, 1.0, interp=interp,
+                 precision="series", device=dev)
+    lay = pl.layout
+    assert lay["precision"] == "series" and lay["bp_series_order"] == 3 and lay["bp_pairs"], lay
+    Y = pl.forward(to_dev(a, dev)).cpu().numpy()
+    X = pl.adjoint(to_dev(y, dev)).cpu().numpy()
+    Yo = oracle_lib.forward(tx, tx, np.zeros_like(tx), pos, a, W, w, fc, fs, N, c_eff=c,
+                            upsample_ratio=U, interp=interp)
+    Xo = oracle_lib.adjoint(tx, tx, np.zeros_like(tx), pos, y, W, w, fc, fs, c_eff=c,
+                            upsample_ratio=U, interp=interp)
+    # a tie decided the other way takes a neighbouring, independent sample: an
+    # error of the order of the pixel's value
+    ex = np.abs(X - Xo) / np.abs(Xo).max()
+    print(f"pair-path ties {interp}: forward {rel_l2(Y, Yo):.2e} adjoint {rel_l2(X, Xo):.2e} "
+          f"(worst pixel {ex.max():.2e})")
+    assert rel_l2(Y, Yo) <= 1e-5 and rel_l2(X, Xo) <= 1e-5 and ex.max() <= 1e-5
+
+
 def test_series_falls_back_when_unbounded(dev, little):
     """At U = 128 on the little problem the series bound still holds (5th order) or
     the plan reports fp64; either way the reported mode is one of the exact ones."""
