@@ -224,28 +224,35 @@ __global__ __launch_bounds__(256) void fill_zero_kernel(float4 *__restrict__ buf
 // is sincospi of the phase reduced in FP64.  One workgroup per valid row: the
 // bins of its span (wrapping at N_w; the bins the gather can read, with
 // margin) and their tail copies, as the spectral stage stores them.
+// AONLY (the nearest rules): A_n alone, rows of `ld` float2 entries.
+template <bool AONLY>
 __global__ __launch_bounds__(256) void pair_rows_kernel(const float2 *__restrict__ prof,
                                                         const int2 *__restrict__ spans,
                                                         const int *__restrict__ rows, int nw,
                                                         int ld, double rho,
-                                                        float4 *__restrict__ pairs) {
+                                                        void *__restrict__ pairs) {
     const int p = rows[blockIdx.x];
     const int2 sp = spans[p];  // first bin in [0, nw), count <= nw
     const float2 *row = prof + (size_t)p * ld;
-    float4 *out = pairs + (size_t)p * ld;
     for (int d = (int)threadIdx.x; d < sp.y; d += 256) {
         int n = sp.x + d;
         n -= n >= nw ? nw : 0;
         for (int i = n; i < ld - 1; i += nw) {
-            const float2 a = row[i], b = row[i + 1];
+            const float2 a = row[i];
             double ph = rho * ((double)i + 0.5);
             ph -= floor(ph);
             double sn, cs;
             sincospi(2.0 * ph, &sn, &cs);
             const double ar = a.x, ai = a.y;
-            const double dr = (double)b.x - ar, di = (double)b.y - ai;
-            out[i] = make_float4((float)(ar * cs + ai * sn), (float)(ai * cs - ar * sn),
-                                 (float)(dr * cs + di * sn), (float)(di * cs - dr * sn));
+            const float2 A = make_float2((float)(ar * cs + ai * sn), (float)(ai * cs - ar * sn));
+            if constexpr (AONLY) {
+                ((float2 *)pairs)[(size_t)p * ld + i] = A;
+            } else {
+                const float2 b = row[i + 1];
+                const double dr = (double)b.x - ar, di = (double)b.y - ai;
+                ((float4 *)pairs)[(size_t)p * ld + i] =
+                    make_float4(A.x, A.y, (float)(dr * cs + di * sn), (float)(di * cs - dr * sn));
+            }
         }
     }
 }
@@ -594,47 +601,44 @@ __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams 
         // padded row: indices beyond it read zeros instead of faulting.
         const i32x4 rsrc = row_pair_rsrc(row, k.ld);
         if constexpr (PAIRS && INTERP != kInterpLinear) {
-            // nearest rules on the demodulated pair rows: the record's cu
-            // carries the rule's offset o (1 for round(1/2 + u), 1/2 for
-            // round(u)), so n = floor(u + o) and f = u + o - n; one 8-byte
-            // load of A_n = h[n] e^{-2 pi i rho (n + 1/2)}, whose pixel phase is
-            // rho f + phi' as on the linear pair path.  Within 2e-7 bin of a
-            // bin edge the reference's rule decides on the exact range
-            // (CPP:346-347, CPP:289), a rare lane-divergent branch.
+            // nearest rules on the demodulated rows (A_n alone, pair_rows_kernel
+            // with AONLY): the record's cu carries the rule's offset o (1 for
+            // round(1/2 + u), 1/2 for round(u)), so n = floor(u + o) and f =
+            // u + o - n; one 8-byte load of A_n = h[n] e^{-2 pi i rho (n + 1/2)},
+            // whose pixel phase is rho f + phi' as on the linear pair path.
+            // Within 2e-7 bin of a bin edge the reference's rule decides on
+            // the exact range (CPP:346-347, CPP:289), a rare lane-divergent
+            // branch.
             static_assert(kHG, "the pair path reads SeriesGather records");
             constexpr bool REF = INTERP == kInterpNearestRef;
-            const i32x4 psrc = pair_rsrc(k.bp_pairs + (size_t)p * k.ld, k.ld);
-            double u[kPixPerThread];
+            const i32x4 psrc = row_pair_rsrc((const float2 *)k.bp_pairs + (size_t)p * k.ld, k.ld);
             int n0[kPixPerThread];
             float f[kPixPerThread];
             bool tie = false;
 #pragma unroll
             for (int m = 0; m < kPixPerThread; ++m) {
                 const double t = fma(g.bx, q[m].dx, fma(g.by, q[m].dy, q[m].d2));
-                u[m] = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
-                n0[m] = (int)u[m];
-                f[m] = (float)__builtin_amdgcn_fract(u[m]);
+                const double u = fma(t, fma(t, fma(t, g.k3, g.k2), g.k1), g.cu);
+                n0[m] = (int)u;
+                f[m] = (float)__builtin_amdgcn_fract(u);
                 tie = tie || fabsf(f[m] - 0.5f) > 0.5f - 2.0f * (float)kTieMargin;
             }
             if (__builtin_amdgcn_read_exec() & __ballot(tie)) {
 #pragma unroll
                 for (int m = 0; m < kPixPerThread; ++m) {
-                    if (fabsf(f[m] - 0.5f) > 0.5f - 2.0f * (float)kTieMargin) {
-                        // the exact range of the pixel (as load_pix places it)
-                        const int i = tile * kTilePixels + m * kThreads + tid;
-                        int ii = i < k.M ? i : k.M - 1;
+                    // (pixels that are not ok contribute nothing: no exact range)
+                    if (q[m].ok && fabsf(f[m] - 0.5f) > 0.5f - 2.0f * (float)kTieMargin) {
+                        // the exact range of the pixel
+                        int i = tile * kTilePixels + m * kThreads + tid;
                         // formed here, not kept in registers across the pulse loop
-                        asm volatile("" : "+v"(ii));
-                        double x = k.px[ii], y = k.py[ii], z = k.pz[ii];
-                        if (!q[m].ok) {
-                            x = X0.x;
-                            y = X0.y;
-                            z = X0.z;
-                        }
+                        asm volatile("" : "+v"(i));
+                        const double x = k.px[i], y = k.py[i], z = k.pz[i];
                         const double ue = ref_dr<MONO>(load_uniform(k.pc, (size_t)p), x, y, z) * k.by_dx;
                         const double ne = round(REF ? 0.5 + ue : ue);
-                        // u - o - ue is the row shift, a multiple of nw, to 1e-9 bin
-                        const int ns = (int)(ne + rint(u[m] - (REF ? 1.0 : 0.5) - ue));
+                        // u - o - ue is the row shift, a multiple of nw (u to
+                        // 1e-7 bin from n0 + f)
+                        const double u = (double)n0[m] + (double)f[m];
+                        const int ns = (int)(ne + rint(u - (REF ? 1.0 : 0.5) - ue));
                         f[m] -= (float)(ns - n0[m]);
                         n0[m] = ns;
                     }
@@ -1568,9 +1572,12 @@ hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s
 }
 
 hipError_t launch_pair_rows(const float2 *prof, const int2 *spans, const int *rows, int Pv,
-                            int nw, int ld, double rho, float4 *pairs, hipStream_t s) {
+                            int nw, int ld, double rho, bool a_only, void *pairs, hipStream_t s) {
     if (Pv <= 0) return hipSuccess;
-    pair_rows_kernel<<<Pv, 256, 0, s>>>(prof, spans, rows, nw, ld, rho, pairs);
+    if (a_only)
+        pair_rows_kernel<true><<<Pv, 256, 0, s>>>(prof, spans, rows, nw, ld, rho, pairs);
+    else
+        pair_rows_kernel<false><<<Pv, 256, 0, s>>>(prof, spans, rows, nw, ld, rho, pairs);
     return hipGetLastError();
 }
 

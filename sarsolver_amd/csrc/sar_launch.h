@@ -112,8 +112,9 @@ hipError_t launch_widen(const float2 *src, double2 *dst, size_t n, hipStream_t s
 hipError_t launch_fill_zero(void *buf, size_t bytes, hipStream_t s);
 // demodulated pairs of the adjoint profile rows over each valid row's span and
 // its tail copies: pairs[i] = (h[i], h[i+1] - h[i]) e^{-2 pi i rho (i + 1/2)}
+// (float4 entries), or with a_only the first of them alone (float2 entries)
 hipError_t launch_pair_rows(const float2 *prof, const int2 *spans, const int *rows, int Pv,
-                            int nw, int ld, double rho, float4 *pairs, hipStream_t s);
+                            int nw, int ld, double rho, bool a_only, void *pairs, hipStream_t s);
 hipError_t launch_gather_perm(const float2 *src, const int *perm, int M, float2 *dst,
                               hipStream_t s);
 hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const PulseConst *pc,

@@ -1049,7 +1049,10 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                                          pl->d_series_g, nullptr));
         }
         if (pl->d_series_g && pl->Pv > 0 && rho_ok)
-            HIP_TRY(hipMalloc((void **)&pl->d_bp_pairs, sizeof(float4) * P * pl->ld));
+            // (the nearest rules read A_n alone: float2 entries)
+            HIP_TRY(hipMalloc((void **)&pl->d_bp_pairs,
+                              (interp == SARHIP_INTERP_LINEAR ? sizeof(float4) : sizeof(float2)) *
+                                  P * pl->ld));
         HIP_TRY(hipDeviceSynchronize());
         owner.p = nullptr;
         *out = pl;
@@ -1393,7 +1396,8 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
                               pl->fft_sparse, s));
         if (pl->d_bp_pairs)
             HIP_TRY(launch_pair_rows(pl->d_prof, pl->d_aspan, pl->d_rows, pl->Pv, (int)pl->nw,
-                                     (int)pl->ld, pl->kc_rev / pl->by_dx, pl->d_bp_pairs, s));
+                                     (int)pl->ld, pl->kc_rev / pl->by_dx,
+                                     pl->interp != SARHIP_INTERP_LINEAR, pl->d_bp_pairs, s));
     } else {
         HIP_TRY(launch_window_pad((const float2 *)phase_history, pl->d_Wconj, pl->d_wfac_adj,
                                   (int)pl->N, (int)pl->nw, (int)pl->ld, (int)pl->P,
