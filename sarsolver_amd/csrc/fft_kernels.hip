@@ -151,21 +151,22 @@ __device__ __forceinline__ void dft(C (&v)[R]) {
 }
 
 // Twiddle tables in LDS, two levels each: e^{-2 pi i t / N} =
-// nlo[t & 63] nhi[t >> 6] (t < N) and e^{-2 pi i t / N_w} = wlo[t & 255]
+// nlo[t & 15] nhi[t >> 4] (t < N) and e^{-2 pi i t / N_w} = wlo[t & 255]
 // whi[t >> 8] (t < N_w); every entry is rounded once from FP64, so an FP32
-// twiddle carries about 1.5 ulp.
+// twiddle carries about 1.5 ulp, and one that is a multiple of 16 (the second
+// radix-16 pass at N >= 4096) is one entry of nhi (0.5 ulp).
 template <typename C>
 struct Twiddles {
     const C *nlo, *nhi, *wlo, *whi;
-    __device__ __forceinline__ C n(const int t) const { return cmul(nhi[t >> 6], nlo[t & 63]); }
+    __device__ __forceinline__ C n(const int t) const { return cmul(nhi[t >> 4], nlo[t & 15]); }
     __device__ __forceinline__ C w(const int t) const { return cmul(whi[t >> 8], wlo[t & 255]); }
 };
-// Host-side layout of the table array: [64 | N/64 | 256 | N_w/256] entries.
+// Host-side layout of the table array: [16 | N/16 | 256 | N_w/256] entries.
 template <int N, typename C>
 __device__ __forceinline__ Twiddles<C> load_twiddles(C *dst, const C *__restrict__ src, int nw) {
-    const int total = 64 + N / 64 + 256 + nw / 256;
+    const int total = 16 + N / 16 + 256 + nw / 256;
     for (int i = threadIdx.x; i < total; i += blockDim.x) dst[i] = src[i];
-    return Twiddles<C>{dst, dst + 64, dst + 64 + N / 64, dst + 64 + N / 64 + 256};
+    return Twiddles<C>{dst, dst + 16, dst + 16 + N / 16, dst + 16 + N / 16 + 256};
 }
 
 // One Stockham pass of radix R with span Ns, in place on the group's LDS
@@ -192,8 +193,10 @@ __device__ __forceinline__ void stockham_pass(C *sh, const Twiddles<C> &tw, int 
         if (j < NB) {
             const int jm = j % Ns;
             if constexpr (Ns > 1) {
+                // pass twiddle stride: a multiple of 16 reads one table entry
+                constexpr int ST = N / (Ns * R);
                 if constexpr (std::is_same<C, double2>::value) {
-                    const C w1 = tw.n(jm * (N / (Ns * R)));
+                    const C w1 = ST % 16 == 0 ? tw.nhi[(jm * ST) >> 4] : tw.n(jm * ST);
                     C w = w1;
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
@@ -203,7 +206,7 @@ __device__ __forceinline__ void stockham_pass(C *sh, const Twiddles<C> &tw, int 
                 } else {
 #pragma unroll
                     for (int r = 1; r < R; ++r) {
-                        const C t = tw.n(jm * r * (N / (Ns * R)));
+                        const C t = ST % 16 == 0 ? tw.nhi[(jm * r * ST) >> 4] : tw.n(jm * r * ST);
                         v[b][r] = cmul(v[b][r], INV ? cconj(t) : t);
                     }
                 }
@@ -608,7 +611,7 @@ constexpr size_t kMaxLds = 160 * 1024;  // LDS of one gfx950 workgroup
 
 #define SARHIP_FFT_SIZES(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192)
 size_t fused_lds_bytes(int N, int U, int Q, size_t elem) {
-    return elem * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + (size_t)U * N / 256);
+    return elem * ((size_t)Q * (N + N / 16) + 16 + N / 16 + 256 + (size_t)U * N / 256);
 }
 
 // ---- 2D spectral filter of the learned preconditioner ----------------------
@@ -802,13 +805,13 @@ hipError_t down(const float2 *prof, const int2 *spans, const float2 *W, const fl
 }
 
 // The forward's instantiations whose buffers fit the LDS for every admitted
-// N_w (tables of at most 64 + N/64 + 256 + 512 entries); the others are never
+// N_w (tables of at most 16 + N/16 + 256 + 512 entries); the others are never
 // selected by fused_fft_phases and are not built.
 template <int N, int Q, typename C>
 hipError_t down_fit(const float2 *prof, const int2 *spans, const float2 *W, const float2 *wfac,
                     int U, int P, int centred, const C *tables, float2 *Y, int *redo,
                     double kappa2, hipStream_t s) {
-    constexpr size_t need = sizeof(C) * ((size_t)Q * (N + N / 16) + 64 + N / 64 + 256 + 512);
+    constexpr size_t need = sizeof(C) * ((size_t)Q * (N + N / 16) + 16 + N / 16 + 256 + 512);
     if constexpr (need <= kMaxLds)
         return down<N, Q, C>(prof, spans, W, wfac, U, P, centred, tables, Y, redo, kappa2, s);
     else
@@ -855,7 +858,7 @@ int fused_fft_phases(int N, int U, bool up, bool sparse, bool fp64) {
 
 // N a power of two in [64, 8192], N_w = U N a multiple of 256 up to 2^17
 // (the two-level twiddle tables stay small; one FP64 phase buffer and the
-// tables then fit the LDS: 151 KiB at N = 8192, N_w = 2^17).
+// tables then fit the LDS: 156 KiB at N = 8192, N_w = 2^17).
 bool fused_fft_supported(int N, int U) {
     const long long nw = (long long)U * N;
     return N >= 64 && N <= 8192 && (N & (N - 1)) == 0 && U >= 1 && nw % 256 == 0 &&

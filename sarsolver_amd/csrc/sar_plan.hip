@@ -992,8 +992,8 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
                     tab64.push_back(make_double2(std::cos(a), std::sin(a)));
                 }
             };
-            put(64, 1, N);
-            put(N / 64, 64, N);
+            put(16, 1, N);
+            put(N / 16, 16, N);
             put(256, 1, nw);
             put(nw / 256, 256, nw);
             std::vector<float2> tab;
