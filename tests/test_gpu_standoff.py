@@ -3,7 +3,9 @@ series argument and coefficients by powers of 1/A^2 (A the pixel-tile to
 platform distance), and the series order follows the plan's error bound.
 A near aperture (~360 m) and a spaceborne-like one (~780 km) hold forward and
 adjoint to the FP64 oracle, monostatic (the third-order record path when the
-bound allows it) and bistatic."""
+bound allows it) and bistatic, with linear interpolation and with the
+reference's nearest-bin rules (which take the same records and, monostatic,
+the demodulated rows)."""
 import numpy as np
 import pytest
 
@@ -20,9 +22,10 @@ def dev():
     return torch.device("cuda", 0)
 
 
+@pytest.mark.parametrize("interp", ["linear", "nearest_ref"])
 @pytest.mark.parametrize("radius,height,bistatic", [
     (300.0, 200.0, False), (600e3, 500e3, False), (600e3, 500e3, True)])
-def test_standoff_matches_oracle(dev, oracle_lib, radius, height, bistatic):
+def test_standoff_matches_oracle(dev, oracle_lib, radius, height, bistatic, interp):
     from sarsolver_amd import SarPlan, synthetic
     P, N = 64, 256
     ap = synthetic.circular_aperture(P, N, 2.0, radius=radius, height=height)
@@ -33,21 +36,23 @@ def test_standoff_matches_oracle(dev, oracle_lib, radius, height, bistatic):
     x = g.standard_normal(sc.num_scatterers) + 1j * g.standard_normal(sc.num_scatterers)
     y = g.standard_normal((P, N)) + 1j * g.standard_normal((P, N))
     W = np.fft.ifftshift(ap.waveform_fft)
-    print(f"case: stand-off radius {radius} m, height {height} m, bistatic={bistatic}", flush=True)
+    print(f"case: stand-off radius {radius} m, height {height} m, bistatic={bistatic}, {interp}",
+          flush=True)
     pl = SarPlan(tx, rx, srp, sc.positions, W, ap.slow_time_weighting, ap.centre_frequency,
-                 ap.sample_bandwidth, ap.c_eff, 8.0, 1.0, interp="linear", precision="series",
+                 ap.sample_bandwidth, ap.c_eff, 8.0, 1.0, interp=interp, precision="series",
                  device=dev)
     Yd = pl.forward(torch.from_numpy(x.astype(np.complex64)).to(dev)).cpu().numpy()
     Xd = pl.adjoint(torch.from_numpy(y.astype(np.complex64)).to(dev)).cpu().numpy()
     Yo = oracle_lib.forward(tx, rx, srp, sc.positions, x, W, ap.slow_time_weighting,
                             ap.centre_frequency, ap.sample_bandwidth, N, upsample_ratio=8.0,
-                            interp="linear")
+                            interp=interp)
     Xo = oracle_lib.adjoint(tx, rx, srp, sc.positions, y, W, ap.slow_time_weighting,
                             ap.centre_frequency, ap.sample_bandwidth, upsample_ratio=8.0,
-                            interp="linear")
+                            interp=interp)
     ef, ea = rel_l2(Yd, Yo), rel_l2(Xd, Xo)
     print(f"  layout {pl.layout}; forward {ef:.2e} adjoint {ea:.2e}")
     if radius > 1e5 and not bistatic:
         # far away the third-order bound holds: the gather reads its records
-        assert pl.layout["bp_series_order"] == 3
+        # and the demodulated rows
+        assert pl.layout["bp_series_order"] == 3 and pl.layout["bp_pairs"]
     assert ef <= 1e-5 and ea <= 1e-5

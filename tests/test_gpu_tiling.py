@@ -3,7 +3,8 @@ the axis along which the platform is seen least, and for strongly directional
 looks that axis is quantised 4x coarser (tiles 4x longer across range).  These
 cases put the look along y, along a diagonal and over a wide aperture (where
 the tiles stay square), monostatic and bistatic, and hold forward and adjoint
-to the FP64 oracle."""
+to the FP64 oracle with linear interpolation and with the reference's
+nearest-bin rules."""
 import numpy as np
 import pytest
 
@@ -20,9 +21,11 @@ def dev():
     return torch.device("cuda", 0)
 
 
+@pytest.mark.parametrize("interp", ["linear", "nearest_ref"])
 @pytest.mark.parametrize("start_deg,aperture_deg,bistatic", [
     (90.0, 3.0, False), (45.0, 3.0, False), (0.0, 100.0, False), (120.0, 4.0, True)])
-def test_look_direction_tiling_matches_oracle(dev, oracle_lib, start_deg, aperture_deg, bistatic):
+def test_look_direction_tiling_matches_oracle(dev, oracle_lib, start_deg, aperture_deg, bistatic,
+                                              interp):
     from sarsolver_amd import SarPlan, synthetic
     P, N = 96, 256
     ap = synthetic.circular_aperture(P, N, aperture_deg, start_deg=start_deg)
@@ -33,18 +36,19 @@ def test_look_direction_tiling_matches_oracle(dev, oracle_lib, start_deg, apertu
     x = g.standard_normal(sc.num_scatterers) + 1j * g.standard_normal(sc.num_scatterers)
     y = g.standard_normal((P, N)) + 1j * g.standard_normal((P, N))
     W = np.fft.ifftshift(ap.waveform_fft)
-    print(f"case: look {start_deg} deg, aperture {aperture_deg} deg, bistatic={bistatic}", flush=True)
+    print(f"case: look {start_deg} deg, aperture {aperture_deg} deg, bistatic={bistatic}, "
+          f"{interp}", flush=True)
     pl = SarPlan(tx, rx, srp, sc.positions, W, ap.slow_time_weighting, ap.centre_frequency,
-                 ap.sample_bandwidth, ap.c_eff, 8.0, 1.0, interp="linear", precision="series",
+                 ap.sample_bandwidth, ap.c_eff, 8.0, 1.0, interp=interp, precision="series",
                  device=dev)
     Yd = pl.forward(torch.from_numpy(x.astype(np.complex64)).to(dev)).cpu().numpy()
     Xd = pl.adjoint(torch.from_numpy(y.astype(np.complex64)).to(dev)).cpu().numpy()
     Yo = oracle_lib.forward(tx, rx, srp, sc.positions, x, W, ap.slow_time_weighting,
                             ap.centre_frequency, ap.sample_bandwidth, N, upsample_ratio=8.0,
-                            interp="linear")
+                            interp=interp)
     Xo = oracle_lib.adjoint(tx, rx, srp, sc.positions, y, W, ap.slow_time_weighting,
                             ap.centre_frequency, ap.sample_bandwidth, upsample_ratio=8.0,
-                            interp="linear")
+                            interp=interp)
     ef, ea = rel_l2(Yd, Yo), rel_l2(Xd, Xo)
     print(f"  layout {pl.layout}; forward {ef:.2e} adjoint {ea:.2e}")
     assert pl.layout["fwd_window_bins"] > 0
