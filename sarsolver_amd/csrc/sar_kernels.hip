@@ -1169,7 +1169,10 @@ __global__ __launch_bounds__(NT) void fp_scatter_kernel(const KernelParams k) {
                         // 1e-7 bin of a tie, where the rule decides on the
                         // exact range (rare, lane-divergent)
                         ln = (int)u;
-                        if (fabs(__builtin_amdgcn_fract(u) - 0.5) > 0.5 - kTieMargin) {
+                        // (the spare records past the image's last pixel add
+                        // zero and have no position record)
+                        if ((!kCompact || ri < k.M) &&
+                            fabs(__builtin_amdgcn_fract(u) - 0.5) > 0.5 - kTieMargin) {
                             double x = px.x, y = px.y, z = px.z;
                             if constexpr (kCompact) {
                                 const PixRec r = load_uniform(k.pixrec, (size_t)ri);
