@@ -199,8 +199,9 @@ int sarhip_plan_info(sarhip_plan_t plan, size_t *working_num_fast_times, double 
  * is monostatic, the number of valid pulses, the forward LDS window length
  * (bins, 0 = global atomics), pulses per forward workgroup, the series
  * truncation bound (bins), the number of 1024-pixel tiles per forward region,
- * the order of the series the linear-interpolation kernels use (3 when its
- * bound meets the same limits, else 5) with its bound, and the upsampling
+ * the order of the series the kernels use (3 when its bound meets the same
+ * limits, else 5; nearest-rule bins within 1e-7 bin of a tie are decided on
+ * the exact range either way) with its bound, and the upsampling
  * ratio U of the fused spectral kernels (N_w = U N; 0 = rocFFT stages).  Any
  * output pointer may be NULL.  No counterpart in the reference. */
 int sarhip_plan_layout(sarhip_plan_t plan, int *precision, int *monostatic, int *valid_pulses,
@@ -222,10 +223,12 @@ int sarhip_plan_forward_fp64_rows(sarhip_plan_t plan, double *kappa, int *rows);
  * stage a pass stores, over the bins the gather can read, (h[n], h[n+1] -
  * h[n]) e^{-2 pi i rho (n + 1/2)} (rho = k_c / (2 pi by_dx), the phase in
  * revolutions per bin), and the gather interpolates with two FMAs and forms
- * each pixel's phase rho f + phi' in FP32 (f the bin fraction).  Enabled for
- * the monostatic planar series plans with fused spectral stages when rho is
- * an FP32 number (to 2^-34) of magnitude at most 2.5; SARHIP_BP_PAIRS=0 at
- * plan creation turns it off.  No counterpart in the reference. */
+ * each pixel's phase rho f + phi' in FP32 (f the bin fraction); for the
+ * nearest-bin rules it stores the first member alone.  Enabled for the
+ * monostatic planar series plans with the third-order series and fused
+ * spectral stages when rho is an FP32 number (to 2^-34) of magnitude at most
+ * 2.5, in every interpolation mode; SARHIP_BP_PAIRS=0 at plan creation turns
+ * it off.  No counterpart in the reference. */
 int sarhip_plan_gather_pairs(sarhip_plan_t plan, int *enabled);
 
 /* Stage timing with HIP events recorded on the caller's stream.  Enabling
