@@ -306,7 +306,8 @@ template <int N, int Q>
 __global__ __launch_bounds__((UpBatch<N, Q>::NT)) __attribute__((amdgpu_waves_per_eu(kUpWaves<N, Q>)))
 void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
                    const float2 *__restrict__ Wconj, const float2 *__restrict__ wfac, int U, int ld,
-                   int centred, const float2 *__restrict__ tables, float2 *__restrict__ prof) {
+                   int centred, const float2 *__restrict__ tables, float2 *__restrict__ prof,
+                   const UpPairs pairs) {
     using Bt = UpBatch<N, Q>;
     constexpr int T = Bt::T;
     extern __shared__ float2 lds[];
@@ -409,6 +410,29 @@ void fft_up_kernel(const int2 *__restrict__ spans, const float2 *__restrict__ Y,
             for (int q = 0; q < Q; ++q)
                 if (in[q])
                     for (int t = nw + n + q; t < ld; t += nw) row[t] = v[q];
+        }
+    }
+    if (pairs.mode != 0) {
+        // the demodulated rows (integer rho: the phasor is the sign) from the
+        // row this workgroup has just stored: its global stores are visible
+        // to the whole workgroup after the barrier.  The same entries as
+        // pair_rows_kernel (span bins and their tail copies), and the same
+        // values: the difference is rounded once, the sign is exact.
+        __syncthreads();
+        const float sg = pairs.sgn;
+        for (int d = tid; d < sp.y; d += Bt::NT) {
+            int n = sp.x + d;
+            n -= n >= nw ? nw : 0;
+            for (int i = n; i < ld - 1; i += nw) {
+                const float2 a = row[i];
+                if (pairs.mode == 2) {
+                    ((float2 *)pairs.out)[(size_t)p * ld + i] = make_float2(sg * a.x, sg * a.y);
+                } else {
+                    const float2 b = row[i + 1];
+                    ((float4 *)pairs.out)[(size_t)p * ld + i] =
+                        make_float4(sg * a.x, sg * a.y, sg * (b.x - a.x), sg * (b.y - a.y));
+                }
+            }
         }
     }
 }
@@ -781,13 +805,14 @@ hipError_t spectral_filter(const float2 *in, const float2 *din, int B, int n, co
 
 template <int N, int Q>
 hipError_t up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac, int U,
-              int ld, int P, int centred, const float2 *tables, float2 *prof, hipStream_t s) {
+              int ld, int P, int centred, const float2 *tables, float2 *prof, UpPairs pairs,
+              hipStream_t s) {
     const size_t lds = fused_lds_bytes(N, U, Q, sizeof(float2));
     hipError_t e = hipFuncSetAttribute((const void *)fft_up_kernel<N, Q>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     fft_up_kernel<N, Q><<<P, UpBatch<N, Q>::NT, lds, s>>>(spans, Y, Wconj, wfac, U, ld, centred,
-                                                           tables, prof);
+                                                           tables, prof, pairs);
     return hipGetLastError();
 }
 
@@ -867,17 +892,20 @@ bool fused_fft_supported(int N, int U) {
 
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
-                         float2 *prof, bool sparse, hipStream_t s) {
+                         float2 *prof, bool sparse, UpPairs pairs, hipStream_t s) {
+    if (pairs.mode != 0 && !spans) return hipErrorInvalidValue;
     const int Q = fused_fft_phases(N, U, true, sparse, false);
-#define UP_CASE(n)                                                                         \
-    if (N == n) {                                                                          \
-        if (Q == 8) return up<n, 8>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        if (Q == 4) return up<n, 4>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        if (Q == 2) return up<n, 2>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s); \
-        return up<n, 1>(spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, s);             \
+#define UP_ARGS spans, Y, Wconj, wfac, U, ld, P, centred, tables, prof, pairs, s
+#define UP_CASE(n)                                     \
+    if (N == n) {                                      \
+        if (Q == 8) return up<n, 8>(UP_ARGS);          \
+        if (Q == 4) return up<n, 4>(UP_ARGS);          \
+        if (Q == 2) return up<n, 2>(UP_ARGS);          \
+        return up<n, 1>(UP_ARGS);                      \
     }
     SARHIP_FFT_SIZES(UP_CASE)
 #undef UP_CASE
+#undef UP_ARGS
     return hipErrorInvalidValue;
 }
 

@@ -148,14 +148,23 @@ hipError_t launch_decode_signal(const void *raw, int format, int nv, int ns, con
 // fused spectral stages (fft_kernels.hip): N-sample band <-> U N-sample rows
 bool fused_fft_supported(int N, int U);
 int fused_fft_phases(int N, int U, bool up, bool sparse, bool fp64);
-// tables: [64 | N/64 | 256 | U N/256] two-level twiddle tables,
-// e^{-2 pi i q / N}, e^{-2 pi i 64 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
+// tables: [16 | N/16 | 256 | U N/256] two-level twiddle tables,
+// e^{-2 pi i q / N}, e^{-2 pi i 16 q / N}, e^{-2 pi i q / N_w}, e^{-2 pi i 256 q / N_w}
 // (tables64: the same in FP64)
 // spans: per-row bins to store {first, count} (and their tail copies), or NULL for whole rows;
 // sparse: the spans cover a small part of the rows (picks the phase batching)
+// Demodulated rows formed by the adjoint spectral kernel itself (an integer
+// rho, whose phasor e^{-2 pi i rho (n + 1/2)} is sgn = (-1)^rho): mode 0 none,
+// 1 the pairs sgn (h[n], h[n+1] - h[n]) (float4 entries), 2 sgn h[n] alone
+// (float2 entries), over the spans (which must be given) and their tails.
+struct UpPairs {
+    void *out;
+    int mode;
+    float sgn;
+};
 hipError_t launch_fft_up(const int2 *spans, const float2 *Y, const float2 *Wconj, const float2 *wfac,
                          int N, int U, int ld, int P, int centred, const float2 *tables,
-                         float2 *prof, bool sparse, hipStream_t s);
+                         float2 *prof, bool sparse, UpPairs pairs, hipStream_t s);
 // spans: per-row covered bins {first, count} (only those are read), or NULL for whole rows;
 // sparse: as for launch_fft_up.  tables64 non-NULL and redo NULL: transforms and
 // phase sums in FP64 for every row; both non-NULL: FP32, then FP64 again for the

@@ -136,6 +136,9 @@ struct sarhip_plan_s {
     double *d_bp_cu = nullptr;  // [num_tiles][Pv] backprojection row offsets of tile centres
     SeriesGather *d_series_g = nullptr;  // [num_tiles][Pv] gather records (see SeriesGather)
     float4 *d_bp_pairs = nullptr;  // [P][ld] demodulated adjoint row pairs (gather pair path)
+    // (-1)^rho when rho is an integer (the adjoint spectral kernel then forms
+    // the demodulated rows itself), else 0 (pair_rows_kernel after it)
+    int pairs_sgn = 0;
     ScatRec *d_srec = nullptr;  // [M] planar scatter records (64-pulse linear series / hilo layout)
     float2 *d_W = nullptr, *d_Wconj = nullptr, *d_wfac_fwd = nullptr, *d_wfac_adj = nullptr;
     float2 *d_prof = nullptr;
@@ -1053,6 +1056,11 @@ int plan_create_impl(sarhip_plan_t *out, size_t num_fast_times, size_t num_slow_
             HIP_TRY(hipMalloc((void **)&pl->d_bp_pairs,
                               (interp == SARHIP_INTERP_LINEAR ? sizeof(float4) : sizeof(float2)) *
                                   P * pl->ld));
+            // an integer rho makes the demodulating phasor a sign, which the
+            // adjoint spectral kernel applies as it stores the row
+            // (SARHIP_PAIRS_FUSED=0: pair_rows_kernel after it instead)
+            if (rho == std::rint(rho) && env_int("SARHIP_PAIRS_FUSED", 1))
+                pl->pairs_sgn = ((long long)std::llround(std::fabs(rho)) % 2 == 0) ? 1 : -1;
         HIP_TRY(hipDeviceSynchronize());
         owner.p = nullptr;
         *out = pl;
@@ -1390,11 +1398,15 @@ int sarhip_adjoint(sarhip_plan_t pl, const void *phase_history, void *amps, int 
     if (pl->fused_U > 0) {
         // band weights, zero padding, inverse FFT and row tails in one pass
         if ((rc = record(pl, 1, s))) return rc;
+        UpPairs up{nullptr, 0, 1.f};
+        if (pl->d_bp_pairs && pl->pairs_sgn != 0)
+            up = UpPairs{pl->d_bp_pairs, pl->interp == SARHIP_INTERP_LINEAR ? 1 : 2,
+                         (float)pl->pairs_sgn};
         HIP_TRY(launch_fft_up(pl->d_aspan, (const float2 *)phase_history, pl->d_Wconj,
                               pl->d_wfac_adj, (int)pl->N, pl->fused_U, (int)pl->ld, (int)pl->P,
                               order == SARHIP_ORDER_CENTRED, pl->d_fft_tables, pl->d_prof,
-                              pl->fft_sparse, s));
-        if (pl->d_bp_pairs)
+                              pl->fft_sparse, up, s));
+        if (pl->d_bp_pairs && pl->pairs_sgn == 0)
             HIP_TRY(launch_pair_rows(pl->d_prof, pl->d_aspan, pl->d_rows, pl->Pv, (int)pl->nw,
                                      (int)pl->ld, pl->kc_rev / pl->by_dx,
                                      pl->interp != SARHIP_INTERP_LINEAR, pl->d_bp_pairs, s));

@@ -56,3 +56,40 @@ def test_pair_path_matches_oracle_and_plain_gather(dev, oracle_lib, fc, fs, U, p
     e1, e0, d = rel_l2(out["1"], Xo), rel_l2(out["0"], Xo), rel_l2(out["1"], out["0"])
     print(f"  rho {fc / (fs * U):.4f}: pair path {e1:.2e}, plain {e0:.2e}, between {d:.2e}")
     assert e1 <= 1e-5 and e0 <= 1e-5 and d <= 1e-6
+
+
+@pytest.mark.parametrize("fc", [9.6e9, 4.8e9])
+@pytest.mark.parametrize("interp", ["linear", "nearest_ref"])
+def test_pairs_formed_by_the_spectral_kernel(dev, oracle_lib, fc, interp):
+    """For an integer rho (2 for the X-band workloads, 1 here with f_c = 4.8 GHz)
+    the demodulating phasor is (-1)^rho, and the adjoint spectral kernel forms
+    the demodulated rows itself as it stores the profile row; with
+    SARHIP_PAIRS_FUSED=0 pair_rows_kernel forms them after it.  The two give
+    the same rows, so the same image bit for bit, and both match the oracle."""
+    from sarsolver_amd import SarPlan, synthetic
+    P, N, U, fs = 128, 512, 8.0, 600e6
+    ap = synthetic.circular_aperture(P, N, 2.0, fc=fc, fs=fs)
+    sc = synthetic.grid_scene(128, 0.1)
+    tx, rx, srp = ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns
+    g = np.random.default_rng(int(fc / 1e8) + 3)
+    y = g.standard_normal((P, N)) + 1j * g.standard_normal((P, N))
+    W = np.fft.ifftshift(ap.waveform_fft)
+    out = {}
+    for flag in ("1", "0"):
+        print(f"case: pairs from the spectral kernel fc {fc} {interp} SARHIP_PAIRS_FUSED={flag}",
+              flush=True)
+        os.environ["SARHIP_PAIRS_FUSED"] = flag
+        try:
+            pl = SarPlan(tx, rx, srp, sc.positions, W, ap.slow_time_weighting, fc, fs, ap.c_eff,
+                         U, 1.0, interp=interp, precision="series", device=dev)
+        finally:
+            os.environ.pop("SARHIP_PAIRS_FUSED", None)
+        assert pl.layout["bp_pairs"], pl.layout
+        out[flag] = pl.adjoint(torch.from_numpy(y.astype(np.complex64)).to(dev)).cpu().numpy()
+        del pl
+    Xo = oracle_lib.adjoint(tx, rx, srp, sc.positions, y, W, ap.slow_time_weighting, fc, fs,
+                            upsample_ratio=U, interp=interp)
+    e = rel_l2(out["1"], Xo)
+    print(f"  rho {fc / (fs * U):.1f}: {e:.2e} against the oracle; identical "
+          f"{np.array_equal(out['1'], out['0'])}")
+    assert np.array_equal(out["1"], out["0"]) and e <= 1e-5
