@@ -124,3 +124,45 @@ def test_adaptive_spectral_stage_switches(dev, scenes):
     assert res["1e-3"][:2] == (1e-3, w.pulses)
     assert res[None][0] == 2.0 and 0 < res[None][1] <= w.pulses
     assert res["1e-3"][2] <= 1e-5 and res[None][2] <= 1e-5
+
+
+@pytest.fixture(scope="module")
+def scenes_ref(oracle_lib, scenes):
+    """The same scenes with the oracle's forward under the reference's bin rule
+    round(u) (CPP:289)."""
+    w, pos, rows, data = scenes
+    out = []
+    for start, ap, x, _ in data:
+        tx = ap.transmitter_positions
+        want = oracle_lib.forward(tx[rows], tx[rows], ap.stab_ref_posns[rows], pos,
+                                  x.astype(np.complex128), np.fft.ifftshift(ap.waveform_fft),
+                                  np.asarray(ap.slow_time_weighting)[rows], ap.centre_frequency,
+                                  ap.sample_bandwidth, w.samples, upsample_ratio=w.upsample,
+                                  interp="nearest_ref", num_threads=15)
+        out.append((start, ap, x, want))
+    return w, pos, rows, out
+
+
+def test_c4_forward_reference_rules(dev, scenes_ref):
+    """The reference's nearest-bin rule on the cancelling scenes (series ranges,
+    window-relative scatter with the exact re-evaluation near ties, adaptive
+    spectral stage), every scene within 1e-5 and normwise 1e-6."""
+    from sarsolver_amd import SarPlan
+    w, pos, rows, data = scenes_ref
+    errs = []
+    for b, (start, ap, x, want) in enumerate(data):
+        print(f"case: c4 forward scene {b} nearest_ref series", flush=True)
+        pl = SarPlan(ap.transmitter_positions, ap.receiver_positions, ap.stab_ref_posns, pos,
+                     np.fft.ifftshift(ap.waveform_fft), ap.slow_time_weighting,
+                     ap.centre_frequency, ap.sample_bandwidth, ap.c_eff, w.upsample, 1.0,
+                     interp="nearest_ref", precision="series", device=dev)
+        with torch.no_grad():
+            got = pl.forward(torch.from_numpy(x).to(dev)).cpu().numpy()[rows]
+        del pl
+        e = rel_l2(got, want)
+        normwise = np.linalg.norm(got - want) / (np.sqrt(ROWS * w.samples) * np.linalg.norm(x))
+        print(f"  start {start:7.1f} deg: rel L2 {e:.2e}, normwise {normwise:.2e}")
+        errs.append(e)
+        assert normwise <= 1e-6
+    print(f"nearest_ref series: max {max(errs):.2e}, median {np.median(errs):.2e}")
+    assert max(errs) <= BOUND["series"]
