@@ -7,8 +7,8 @@ decide whether to stop (both solvers are written the same way, so their
 wall times compare the methods and not the bookkeeping).
 
 * `cgls`: conjugate gradients on the normal equations written with A and A^H
-  (one A and one A^H per iteration).  It is the fallback and the judge: every
-  solve is scored by `true_residual`, recomputed from scratch.
+  (one A and one A^H per iteration).  It is the fallback and the yardstick:
+  every solve is scored by `true_residual`, recomputed from scratch.
 * `fpcg`: flexible PCG with a preconditioner callable z = M(r) that may be
   nonlinear (a trained network): Polak-Ribiere beta.  Non-finite values,
   non-positive curvature p^H K p <= 0, or a residual grown to 10x the
