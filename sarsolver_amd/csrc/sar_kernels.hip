@@ -540,9 +540,20 @@ template <int INTERP, bool MONO, int PREC, bool PLANAR, int SORD = 5, bool PAIRS
 __global__ __launch_bounds__(kThreads) void bp_gather_kernel(const KernelParams k) {
     __shared__ double acc[2 * kPixPerThread][kThreads];
     const PolyRegs pr = make_poly_regs<SORD>();
-    // grid: (tile, pulse split); split s takes the valid pulses [j_begin, j_end)
-    const int tile = blockIdx.x;
-    const int split = blockIdx.y, nsplit = (int)gridDim.y;
+    // grid: one workgroup per (tile, pulse split), split s taking the valid
+    // pulses [j_begin, j_end).  XCD-aware order: workgroups are dealt
+    // round-robin to the 8 XCDs, so XCD x gets the x-th eighth of the pairs
+    // in (split, tile) order, tiles in Morton order: the workgroups an XCD
+    // runs together are neighbouring tiles on the same pulses, whose profile
+    // bins its L2 then serves to all of them.  (Blocks past the last pair
+    // exit.)
+    const int nsplit = k.bp_splits;
+    const long long npairs = (long long)k.num_tiles * nsplit;
+    const long long chunk = (npairs + 7) >> 3;
+    const long long f = (long long)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+    if (f >= npairs) return;
+    const int tile = (int)(f % k.num_tiles);
+    const int split = (int)(f / k.num_tiles);
     const int j_begin = (int)((long long)k.Pv * split / nsplit);
     const int j_end = (int)((long long)k.Pv * (split + 1) / nsplit);
     const int tid = (int)threadIdx.x;
@@ -1733,7 +1744,8 @@ hipError_t launch_series_consts(const double4 *tile_c, int num_tiles, const Puls
 
 template <int I, bool MO>
 static void bp_dispatch_prec(const KernelParams &k, hipStream_t s) {
-    const dim3 grid(k.num_tiles, k.bp_splits);
+    // (tile, split) pairs in the kernel's XCD-aware order
+    const dim3 grid((unsigned)(8 * (((long long)k.num_tiles * k.bp_splits + 7) / 8)));
     // planar specialisations for the linear series / hilo paths only
     const bool pl = I == kInterpLinear && k.planar;
     switch (k.precision) {
